@@ -1,0 +1,209 @@
+"""Python access to the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; the product never uses it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "agbnp3_ref_driver")
+ENERGY_NAMES = ("egb", "evdw", "ecorr_vdw", "ecav", "ecorr_cav", "ehb", "mol_volume")
+HB_FIXED = 1
+SPLINE_FIXED = 2
+
+_lib = None
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle oracle`")
+        L = ctypes.CDLL(LIB)
+        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.oracle_new.restype = vp
+        L.oracle_new.argtypes = [ci, vp, vp, vp, vp, vp, cd, cd, vp, vp, vp, vp, vp, vp, ci, vp, vp, vp]
+        L.oracle_free.argtypes = [vp]
+        L.oracle_ener.restype = ci
+        L.oracle_ener.argtypes = [vp, vp, vp, vp, ci, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_natoms.argtypes = [vp]
+        L.oracle_nheavy.argtypes = [vp]
+        L.oracle_int2ext.argtypes = [vp, vp]
+        L.oracle_near_list.argtypes = [vp, vp, vp, ci]
+        L.oracle_tables.argtypes = [vp, vp, vp, vp]
+        L.oracle_rtype.argtypes = [vp, vp]
+        L.oracle_internal.argtypes = [vp, ctypes.c_char_p, vp]
+        L.oracle_counts.argtypes = [vp, vp]
+        L.oracle_water_sites.argtypes = [vp, vp, vp, vp, vp]
+        L.oracle_i4.restype = cd
+        L.oracle_i4.argtypes = [cd, cd, cd, ctypes.POINTER(cd)]
+        L.oracle_i4ov_f32.restype = ctypes.c_float
+        L.oracle_i4ov_f32.argtypes = [ctypes.c_float] * 3 + [ctypes.POINTER(ctypes.c_float)]
+        L.oracle_ogauss.restype = cd
+        L.oracle_ogauss.argtypes = [cd, cd, vp, cd, cd, vp, vp]
+        L.oracle_q4.restype = cd
+        L.oracle_q4.argtypes = [vp, ci, cd, cd, ctypes.POINTER(cd)]
+        L.oracle_place_ws.restype = ci
+        L.oracle_place_ws.argtypes = [vp, ci, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """agbnp3_new()/agbnp3_ener() of the FP64 restatement for one system."""
+
+    def __init__(self, s):
+        L = load()
+        self.sys = s
+        self._keep = [np.ascontiguousarray(a) for a in (
+            s.x, s.y, s.z, s.r, s.charge, s.igamma, s.sgamma, s.ialpha, s.salpha)]
+        self._ikeep = [np.ascontiguousarray(a, np.int32) for a in (s.hbtype, s.ihydrogen, s.conn_nne,
+                                                                    s.conn_neighl1)]
+        hbcorr = np.ascontiguousarray(s.hbcorr, np.float64)
+        k, ik = self._keep, self._ikeep
+        self.h = L.oracle_new(s.natoms, _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), s.eps_in, s.eps_out,
+                              _p(k[5]), _p(k[6]), _p(k[7]), _p(k[8]), _p(ik[0]), _p(hbcorr),
+                              s.nhydrogen, _p(ik[1]), _p(ik[2]), _p(ik[3]))
+        if not self.h:
+            raise RuntimeError("oracle_new failed")
+        self.n = s.natoms
+        self.nheavy = L.oracle_nheavy(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.oracle_free(self.h)
+            self.h = None
+
+    def ener(self, x=None, y=None, z=None, hb_fixed=False, spline_fixed=False):
+        L = load()
+        n = self.n
+        x = np.ascontiguousarray(self.sys.x if x is None else x, np.float64)
+        y = np.ascontiguousarray(self.sys.y if y is None else y, np.float64)
+        z = np.ascontiguousarray(self.sys.z if z is None else z, np.float64)
+        e = np.zeros(7)
+        sp, br, sa = np.zeros(n), np.zeros(n), np.zeros(n)
+        g = [np.zeros((n, 3)) for _ in range(4)]
+        flags = (HB_FIXED if hb_fixed else 0) | (SPLINE_FIXED if spline_fixed else 0)
+        rc = L.oracle_ener(self.h, _p(x), _p(y), _p(z), flags, _p(e), _p(sp), _p(br),
+                           _p(sa), _p(g[0]), _p(g[1]), _p(g[2]), _p(g[3]))
+        if rc != 0:
+            raise RuntimeError("oracle_ener failed")
+        out = dict(zip(ENERGY_NAMES, e))
+        out.update(sp=sp, br=br, surf_area=sa, dgbdr=g[0], dvwdr=g[1], decav=g[2], dehb=g[3])
+        return out
+
+    def near_list(self):
+        """(nne[nheavy], padded rows[nheavy, cap]) in internal heavy-first order."""
+        L = load()
+        nne = np.zeros(self.nheavy, np.int32)
+        cap = L.oracle_near_list(self.h, _p(nne), None, 0)
+        cap = max(cap, 1)
+        rows = np.full((self.nheavy, cap), -1, np.int32)
+        L.oracle_near_list(self.h, _p(nne), _p(rows), cap)
+        return nne, rows
+
+    def int2ext(self):
+        out = np.zeros(self.n, np.int32)
+        load().oracle_int2ext(self.h, _p(out))
+        return out
+
+    def tables(self):
+        L = load()
+        nt = L.oracle_tables(self.h, None, None, None)
+        y = np.zeros(nt * nt * 512, np.float32)
+        y2 = np.zeros_like(y)
+        radii = np.zeros(nt, np.float32)
+        L.oracle_tables(self.h, _p(y), _p(y2), _p(radii))
+        return y.reshape(nt * nt, 512), y2.reshape(nt * nt, 512), radii
+
+    def rtype(self):
+        out = np.zeros(self.n, np.int32)
+        load().oracle_rtype(self.h, _p(out))
+        return out
+
+    def internal(self, name):
+        out = np.zeros(self.n)
+        if load().oracle_internal(self.h, name.encode(), _p(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def counts(self):
+        c = np.zeros(4, np.int64)
+        load().oracle_counts(self.h, _p(c))
+        return dict(kept_overlaps=int(c[0]), evaluated_overlaps=int(c[1]), water_sites=int(c[2]),
+                    near_pairs=int(c[3]))
+
+    def water_sites(self):
+        L = load()
+        nws = L.oracle_water_sites(self.h, None, None, None, None)
+        pos = np.zeros((nws, 3))
+        par = np.zeros(nws, np.int32)
+        typ = np.zeros(nws, np.int32)
+        sp = np.zeros(nws)
+        L.oracle_water_sites(self.h, _p(pos), _p(par), _p(typ), _p(sp))
+        return dict(pos=pos, parent0=par, type=typ, sp=sp)
+
+    def q4(self, table, d, Rj, spline_fixed=False):
+        self.ener(spline_fixed=spline_fixed)   # sets the derivative convention
+        dq = ctypes.c_double(0.0)
+        q = load().oracle_q4(self.h, int(table), float(d), float(Rj), ctypes.byref(dq))
+        return q, dq.value
+
+    def place_ws(self, iat, xyz_int):
+        X = np.ascontiguousarray(xyz_int, np.float64).reshape(-1)
+        pos = np.zeros((4, 3))
+        dpos = np.zeros((4, 4, 3, 3))
+        par = np.zeros((4, 4), np.int32)
+        npar = np.zeros(4, np.int32)
+        ns = load().oracle_place_ws(self.h, int(iat), _p(X), _p(pos), _p(dpos), _p(par), _p(npar))
+        return ns, pos[:max(ns, 0)], dpos[:max(ns, 0)], par[:max(ns, 0)], npar[:max(ns, 0)]
+
+
+def i4(rij, Ri, Rj):
+    dr = ctypes.c_double(0.0)
+    q = load().oracle_i4(float(rij), float(Ri), float(Rj), ctypes.byref(dr))
+    return q, dr.value
+
+
+def i4ov_f32(rij, Ri, Rj):
+    dr = ctypes.c_float(0.0)
+    q = load().oracle_i4ov_f32(float(rij), float(Ri), float(Rj), ctypes.byref(dr))
+    return q, dr.value
+
+
+def ogauss(a1, p1, c1, a2, p2, c2):
+    c1 = np.ascontiguousarray(c1, np.float64)
+    c2 = np.ascontiguousarray(c2, np.float64)
+    out = np.zeros(8)
+    v = load().oracle_ogauss(a1, p1, _p(c1), a2, p2, _p(c2), _p(out))
+    return v, out
+
+
+def run_reference(system, workdir, repeat=1, warmup=0, threads=None, timeout=None):
+    """Run the reference library (oracle/_ref) on `system`; returns (results, timing)."""
+    import json
+    import subprocess
+    from agbnp3_amd import sysio
+    if not os.path.exists(REF_DRIVER):
+        raise FileNotFoundError(REF_DRIVER)
+    os.makedirs(workdir, exist_ok=True)
+    fin = os.path.join(workdir, "in.sys")
+    fout = os.path.join(workdir, "out.bin")
+    sysio.write_system(fin, system)
+    env = dict(os.environ)
+    if threads is not None:
+        env["OMP_NUM_THREADS"] = str(threads)
+    p = subprocess.run([REF_DRIVER, fin, fout, "--repeat", str(repeat), "--warmup", str(warmup)],
+                       capture_output=True, text=True, env=env, timeout=timeout)
+    if p.returncode != 0:
+        raise RuntimeError(f"reference driver failed: {p.stderr[-2000:]}")
+    timing = json.loads(p.stdout.strip().splitlines()[-1])
+    return sysio.read_results(fout, system.natoms), timing

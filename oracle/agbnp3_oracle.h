@@ -1,0 +1,94 @@
+/* FP64 CPU oracle for the AGBNP3 energy + gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This restatement is the checker for the HIP
+ * implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (libagbnp3) never calls it.
+ *
+ * Every function restates a reference routine (file:line in its comment) in
+ * double precision, fed with the same FP32-rounded inputs the reference uses
+ * (coordinates, radii R = (float)(r + 0.5), charges and parameters) and the
+ * same FP32 Q4 spline tables.  The neighbour list is evaluated in FP32 exactly
+ * as the reference does, so it is an integer-exact oracle for that stage.
+ */
+#ifndef AGBNP3_ORACLE_H
+#define AGBNP3_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* flags */
+#define ORACLE_HB_FIXED 1   /* E_HB free volume uses the current atom's spe
+                               (default reproduces the reference's stale index) */
+#define ORACLE_SPLINE_FIXED 2 /* spline derivative with the textbook sign of the
+                                 y''_{k+1} term (default reproduces the reference,
+                                 agbnp3_utils.c:292-293, whose dQ4 is not the
+                                 derivative of Q4) */
+
+/* energy slots */
+enum { ORC_EGB = 0, ORC_EVDW, ORC_ECORR_VDW, ORC_ECAV, ORC_ECORR_CAV, ORC_EHB,
+       ORC_MOLVOL, ORC_NENERGY };
+
+typedef struct OracleSys OracleSys;
+
+/* agbnp3_new analogue (agbnp3.c:121-419): deep copy + heavy-first remap,
+   radius types and FP32 spline tables. conntbl given as nne[] + flat list.
+   Returns NULL on error (message on stderr). */
+OracleSys *oracle_new(int natoms, const double *x, const double *y, const double *z,
+                      const double *r, const double *charge,
+                      double eps_in, double eps_out,
+                      const double *igamma, const double *sgamma,
+                      const double *ialpha, const double *salpha,
+                      const int *hbtype, const double *hbcorr,
+                      int nhydrogen, const int *ihydrogen,
+                      const int *conn_nne, const int *conn_neighl1);
+void oracle_free(OracleSys *s);
+
+/* agbnp3_ener analogue (agbnp3.c:492-566). Outputs in caller order.
+   energies[ORC_NENERGY]; per-atom arrays natoms; gradients 3*natoms.
+   Returns 0 on success, -1 on error. */
+int oracle_ener(OracleSys *s, const double *x, const double *y, const double *z,
+                int flags, double *energies, double *sp, double *br, double *surf_area,
+                double *dgbdr, double *dvwdr, double *decav, double *dehb);
+
+/* introspection of the last oracle_ener call (internal atom order) */
+int oracle_natoms(const OracleSys *s);
+int oracle_nheavy(const OracleSys *s);
+void oracle_int2ext(const OracleSys *s, int *int2ext);
+/* near list of the last call: nne[nheavy], rows packed at offsets cap*i */
+int oracle_near_list(const OracleSys *s, int *nne, int *nbr, int cap);
+/* spline tables: ntypes returned; y/y2 sized ntypes*ntypes*512 (may be NULL) */
+int oracle_tables(const OracleSys *s, float *y, float *y2, float *radii);
+int oracle_rtype(const OracleSys *s, int *rtype);
+/* named internal per-atom array (internal order, natoms doubles):
+   "volumep_raw" "sa_raw" "sa_f" "sp" "spe" "psvol" "gammap" "beta" "br"
+   "brw" "swf" "A" "q2ab" "abrw" "deru" "derv" "derus" "dervs" "derh" */
+int oracle_internal(const OracleSys *s, const char *name, double *out);
+/* counts: [0] kept overlaps, [1] evaluated overlaps, [2] water sites,
+   [3] near-list entries */
+void oracle_counts(const OracleSys *s, long *counts);
+/* water sites of the last call: positions (3*nws), parent atom (internal),
+   type, khb, free-volume fraction */
+int oracle_water_sites(const OracleSys *s, double *pos, int *parent0, int *type,
+                       double *sp);
+
+/* ---- closed-form pieces, for unit tests ---- */
+/* agbnp3_i4 (agbnp3.c:3886-3944), double */
+double oracle_i4(double rij, double Ri, double Rj, double *dr);
+/* agbnp3_i4ov in the reference's FP32 arithmetic (agbnp3.c:3964-3975) */
+float oracle_i4ov_f32(float rij, float Ri, float Rj, float *dr);
+/* Gaussian pair overlap (agbnp3_cpu_simd_kernels.c:2418-2477), double:
+   returns V; out[0..5] = a12, p12, c12x, c12y, c12z, V' ; out[6]=fp, out[7]=fpp */
+double oracle_ogauss(double a1, double p1, const double *c1,
+                     double a2, double p2, const double *c2, double *out);
+/* Q4 lookup: table t at distance d with descreener radius Rj */
+double oracle_q4(const OracleSys *s, int table, double d, double Rj, double *dq4);
+/* water-site placement for atom iat (internal index) at the current
+   coordinates: returns number of sites, -1 on the TRIGONAL_OOP error.
+   pos[4][3], dpos[4][4][3][3], parents[4][4], nparents[4] */
+int oracle_place_ws(const OracleSys *s, int iat, const double *xyz_int,
+                    double *pos, double *dpos, int *parents, int *nparents);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,77 @@
+"""The config-0 generator: determinism, composition, packing, HB topology."""
+import numpy as np
+
+from agbnp3_amd import synth
+
+
+def test_deterministic_for_a_seed():
+    a = synth.generate(20260001, 2000)
+    b = synth.generate(20260001, 2000)
+    for k in ("x", "y", "z", "r", "charge", "ialpha", "hbcorr"):
+        assert np.array_equal(getattr(a, k), getattr(b, k))
+    assert np.array_equal(a.conn_neighl1, b.conn_neighl1)
+    c = synth.generate(20260002, 2000)
+    assert not np.array_equal(a.x, c.x)
+
+
+def test_composition_density_and_charges():
+    s = synth.generate(20260001, 2000)
+    assert s.natoms == 2000
+    frac = {r: np.mean(s.r == r) for r in (1.7, 1.55, 1.52, 1.8, 1.2)}
+    assert abs(frac[1.2] - 0.50) < 0.04           # H ~50%
+    assert abs(frac[1.7] - 0.32) < 0.04           # C 32%
+    assert abs(frac[1.55] - 0.08) < 0.03          # N 8%
+    assert abs(frac[1.52] - 0.09) < 0.03          # O 9%
+    assert frac[1.8] < 0.03                       # S ~1%
+    assert abs(s.charge.sum()) < 1e-9
+    R = s.info["sphere_radius"]
+    assert abs(R - (3 * 2000 / (4 * np.pi * 0.1)) ** (1 / 3)) < 0.5
+    heavy = np.ones(s.natoms, bool)
+    heavy[s.ihydrogen] = False
+    assert np.all(s.igamma[heavy] == 0.117) and np.all(s.igamma[~heavy] == 0)
+    assert np.all((s.ialpha[heavy] >= -0.6) & (s.ialpha[heavy] <= -0.2))
+
+
+def test_nonbonded_exclusion_and_bonds():
+    from scipy.spatial import cKDTree
+    s = synth.generate(20260001, 2000)
+    X = np.stack([s.x, s.y, s.z], 1)
+    off = np.concatenate([[0], np.cumsum(s.conn_nne)])
+    nb = [set(s.conn_neighl1[off[i]:off[i + 1]]) for i in range(s.natoms)]
+    heavy = np.ones(s.natoms, bool)
+    heavy[s.ihydrogen] = False
+    for i in range(s.natoms):                  # symmetric bond graph with sane lengths
+        for j in nb[i]:
+            assert i in nb[j]
+            d = np.linalg.norm(X[i] - X[j])
+            assert abs(d - (1.53 if heavy[i] and heavy[j] else 1.0)) < 1e-6
+    hv = np.nonzero(heavy)[0]
+    t = cKDTree(X[hv])
+    for a, b in t.query_pairs(2.5 - 1e-9):
+        i, j = hv[a], hv[b]
+        one3 = j in nb[i] or any(j in nb[k] for k in nb[i])
+        assert one3, (i, j)
+
+
+def test_hb_topologies_are_complete():
+    s = synth.generate(20260001, 2000)
+    off = np.concatenate([[0], np.cumsum(s.conn_nne)])
+    nne = s.conn_nne
+    ish = np.zeros(s.natoms, bool)
+    ish[s.ihydrogen] = True
+    kinds = set()
+    for i in np.nonzero(s.hbtype)[0]:
+        t, m = s.hbtype[i], nne[i]
+        nbrs = s.conn_neighl1[off[i]:off[i + 1]]
+        if t == 1:
+            assert ish[i] and any(~ish[nbrs])
+        elif t in (10, 30) and m == 1:
+            assert nne[nbrs[0]] == 3
+        elif t == 13:
+            assert m == 3
+        elif t == 20 and m == 1:
+            assert nne[nbrs[0]] == 4
+        else:
+            assert (t, m) in ((10, 2), (20, 2), (20, 3)), (t, m)
+        kinds.add((int(t), int(m)))
+    assert kinds >= {(1, 1), (10, 1), (10, 2), (13, 3), (20, 1), (20, 2), (20, 3), (30, 1)}
