@@ -1,0 +1,257 @@
+"""Python bindings of libagbnp3 through its C-ABI (ctypes).
+
+Mirrors the reference API of agbnp3.h (agbnp3_initialize, agbnp3_new,
+agbnp3_ener, agbnp3_delete, agbnp3_terminate and the NeighList struct of
+nblist.h) with the same argument meaning and AGBNP_OK / AGBNP_ERR returns,
+plus the extensions of include/agbnp3_amd.h.  There is no CPU fallback:
+loading fails loudly when libagbnp3.so is missing, and agbnp3_initialize
+returns AGBNP_ERR when no GPU is visible.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _paths
+
+AGBNP_OK = 2
+AGBNP_ERR = -1
+OPT_HB_FIXED = 1
+OPT_SPLINE_FIXED = 2
+OPT_TIMING = 4
+ENERGY_NAMES = ("egb", "evdw", "ecorr_vdw", "ecav", "ecorr_cav", "ehb", "mol_volume")
+
+# exported C symbols, as declared in include/agbnp3.h and include/agbnp3_amd.h
+EXPORTED = (
+    "agbnp3_initialize", "agbnp3_terminate", "agbnp3_new", "agbnp3_delete", "agbnp3_ener",
+    "agbnp_initialize", "agbnp_terminate", "agbnp_delete",
+    "agbnp3_set_options", "agbnp3_ener_device", "agbnp3_sync", "agbnp3_stream", "agbnp3_natoms",
+    "agbnp3_get_int2ext", "agbnp3_get_near_list", "agbnp3_get_tables", "agbnp3_get_internal",
+    "agbnp3_get_counts", "agbnp3_kernel_times",
+)
+NBLIST_EXPORTED = ("nblist_reset_neighbor_list", "nblist_delete_neighbor_list", "nblist_reallocate_neighbor_list")
+
+
+class NeighVector(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double)]
+
+
+class NeighList(ctypes.Structure):
+    """ABI of nblist.h:39-70."""
+    _fields_ = [
+        ("natoms", ctypes.c_int), ("neighl_size", ctypes.c_int),
+        ("nne", ctypes.POINTER(ctypes.c_int)), ("neighl1", ctypes.POINTER(ctypes.c_int)),
+        ("neighl", ctypes.POINTER(ctypes.POINTER(ctypes.c_int))),
+        ("idx_remap", ctypes.c_int),
+        ("int2ext", ctypes.POINTER(ctypes.c_int)), ("ext2int", ctypes.POINTER(ctypes.c_int)),
+        ("pbc", ctypes.c_int),
+        ("pbc_trans1", ctypes.POINTER(NeighVector)), ("pbc_trans", ctypes.POINTER(ctypes.POINTER(NeighVector))),
+        ("data", ctypes.c_int),
+        ("data_index1", ctypes.POINTER(ctypes.c_void_p)), ("data_index", ctypes.POINTER(ctypes.c_void_p)),
+    ]
+
+
+_lib = None
+_nb = None
+
+
+def load():
+    """Load libagbnp3.so (and libnblist.so); raises if they are not built."""
+    global _lib, _nb
+    if _lib is None:
+        for p in (_paths.LIBNBLIST, _paths.LIBAGBNP3):
+            if not os.path.exists(p):
+                raise RuntimeError(f"{p} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        _nb = ctypes.CDLL(_paths.LIBNBLIST, mode=ctypes.RTLD_GLOBAL)
+        L = ctypes.CDLL(_paths.LIBAGBNP3)
+        vp, ci, dp, ip = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        L.agbnp3_initialize.restype = ci
+        L.agbnp3_terminate.restype = None
+        L.agbnp3_new.restype = ci
+        L.agbnp3_new.argtypes = [ctypes.POINTER(ci), ci, vp, vp, vp, vp, vp, dp, dp, vp, vp, vp, vp, ip, vp, ci, ip,
+                                 ctypes.POINTER(NeighList), ci]
+        L.agbnp3_delete.argtypes = [ci]
+        L.agbnp3_ener.restype = ci
+        L.agbnp3_ener.argtypes = [ci, ci] + [vp] * 17
+        L.agbnp3_set_options.argtypes = [ci, ci]
+        L.agbnp3_ener_device.restype = ci
+        L.agbnp3_ener_device.argtypes = [ci] + [vp] * 11
+        L.agbnp3_sync.argtypes = [ci]
+        L.agbnp3_stream.restype = vp
+        L.agbnp3_stream.argtypes = [ci]
+        L.agbnp3_natoms.argtypes = [ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]
+        L.agbnp3_get_int2ext.argtypes = [ci, vp]
+        L.agbnp3_get_near_list.argtypes = [ci, vp, vp, ci]
+        L.agbnp3_get_tables.argtypes = [ci, vp, vp, vp]
+        L.agbnp3_get_internal.argtypes = [ci, ctypes.c_char_p, vp]
+        L.agbnp3_get_counts.argtypes = [ci, vp]
+        L.agbnp3_kernel_times.argtypes = [ci, vp, ci, ctypes.c_char_p, ci]
+        _nb.nblist_reallocate_neighbor_list.argtypes = [ctypes.POINTER(NeighList), ci, ci]
+        _nb.nblist_reset_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
+        _nb.nblist_delete_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_conntbl(conn_nne, conn_neighl1):
+    """A NeighList filled through libnblist, as a caller would build it."""
+    load()
+    nl = NeighList()
+    _nb.nblist_reset_neighbor_list(ctypes.byref(nl))
+    n = int(len(conn_nne))
+    size = max(int(len(conn_neighl1)), 1)
+    if _nb.nblist_reallocate_neighbor_list(ctypes.byref(nl), n, size) != 0:
+        raise RuntimeError("nblist_reallocate_neighbor_list failed")
+    base = ctypes.addressof(nl.neighl1.contents)
+    off = 0
+    for i in range(n):
+        nl.nne[i] = int(conn_nne[i])
+        nl.neighl[i] = ctypes.cast(base + 4 * off, ctypes.POINTER(ctypes.c_int))
+        for k in range(int(conn_nne[i])):
+            nl.neighl1[off + k] = int(conn_neighl1[off + k])
+        off += int(conn_nne[i])
+    return nl
+
+
+# ---------------------------------------------------------- reference API
+def agbnp3_initialize():
+    return load().agbnp3_initialize()
+
+
+def agbnp3_terminate():
+    load().agbnp3_terminate()
+
+
+def agbnp3_new(natoms, x, y, z, r, charge, dielectric_in, dielectric_out, igamma, sgamma, ialpha, salpha,
+               hbtype, hbcorr, nhydrogen, ihydrogen, conntbl, verbose=0):
+    """Returns (rc, tag)."""
+    L = load()
+    d = [np.ascontiguousarray(a, np.float64) for a in (x, y, z, r, charge, igamma, sgamma, ialpha, salpha, hbcorr)]
+    hb = np.ascontiguousarray(hbtype, np.int32)
+    ih = np.ascontiguousarray(ihydrogen, np.int32)
+    tag = ctypes.c_int(-1)
+    rc = L.agbnp3_new(ctypes.byref(tag), int(natoms), _p(d[0]), _p(d[1]), _p(d[2]), _p(d[3]), _p(d[4]),
+                      float(dielectric_in), float(dielectric_out), _p(d[5]), _p(d[6]), _p(d[7]), _p(d[8]),
+                      _p(hb), _p(d[9]), int(nhydrogen), _p(ih), ctypes.byref(conntbl) if conntbl is not None else None,
+                      int(verbose))
+    return rc, tag.value
+
+
+def agbnp3_ener(tag, init, x, y, z):
+    """Returns (rc, dict of outputs) with arrays in caller order."""
+    L = load()
+    x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+    n = x.size
+    sp, br, sa = np.zeros(n), np.zeros(n), np.zeros(n)
+    g = [np.zeros((n, 3)) for _ in range(4)]
+    e = [ctypes.c_double(0.0) for _ in range(7)]
+    rc = L.agbnp3_ener(int(tag), int(init), _p(x), _p(y), _p(z), _p(sp), _p(br), ctypes.byref(e[6]), _p(sa),
+                       ctypes.byref(e[0]), _p(g[0]), ctypes.byref(e[1]), ctypes.byref(e[2]), _p(g[1]),
+                       ctypes.byref(e[3]), ctypes.byref(e[4]), _p(g[2]), ctypes.byref(e[5]), _p(g[3]))
+    out = dict(zip(ENERGY_NAMES, (v.value for v in e)))
+    out.update(sp=sp, br=br, surf_area=sa, dgbdr=g[0], dvwdr=g[1], decav=g[2], dehb=g[3])
+    return rc, out
+
+
+def agbnp3_delete(tag):
+    return load().agbnp3_delete(int(tag))
+
+
+agbnp_initialize = agbnp3_initialize
+agbnp_terminate = agbnp3_terminate
+agbnp_delete = agbnp3_delete
+
+
+# ------------------------------------------------------------ convenience
+class Instance:
+    """One agbnp3_new() instance built from a synth.System."""
+
+    def __init__(self, s, options=0):
+        if agbnp3_initialize() != AGBNP_OK:
+            raise RuntimeError("agbnp3_initialize failed (no GPU?)")
+        self.sys = s
+        self.conn = make_conntbl(s.conn_nne, s.conn_neighl1)
+        rc, tag = agbnp3_new(s.natoms, s.x, s.y, s.z, s.r, s.charge, s.eps_in, s.eps_out, s.igamma, s.sgamma,
+                             s.ialpha, s.salpha, s.hbtype, s.hbcorr, s.nhydrogen, s.ihydrogen, self.conn)
+        if rc != AGBNP_OK:
+            raise RuntimeError("agbnp3_new failed")
+        self.tag = tag
+        self.n = s.natoms
+        if options:
+            self.set_options(options)
+
+    def set_options(self, options):
+        if load().agbnp3_set_options(self.tag, int(options)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_set_options failed")
+
+    def ener(self, x=None, y=None, z=None):
+        s = self.sys
+        rc, out = agbnp3_ener(self.tag, 0, s.x if x is None else x, s.y if y is None else y,
+                              s.z if z is None else z)
+        if rc != AGBNP_OK:
+            raise RuntimeError("agbnp3_ener failed")
+        return out
+
+    def delete(self):
+        if self.tag is not None:
+            agbnp3_delete(self.tag)
+            self.tag = None
+            _nb.nblist_delete_neighbor_list(ctypes.byref(self.conn))
+
+    def nheavy(self):
+        n, nh = ctypes.c_int(), ctypes.c_int()
+        load().agbnp3_natoms(self.tag, ctypes.byref(n), ctypes.byref(nh))
+        return nh.value
+
+    def int2ext(self):
+        out = np.zeros(self.n, np.int32)
+        load().agbnp3_get_int2ext(self.tag, _p(out))
+        return out
+
+    def near_list(self, cap=256):
+        nh = self.nheavy()
+        nne = np.zeros(max(nh, 1), np.int32)
+        nbr = np.full((max(nh, 1), cap), -1, np.int32)
+        load().agbnp3_get_near_list(self.tag, _p(nne), _p(nbr), cap)
+        return nne[:nh], nbr[:nh]
+
+    def tables(self):
+        L = load()
+        nt = L.agbnp3_get_tables(self.tag, None, None, None)
+        y = np.zeros(nt * nt * 512, np.float32)
+        y2 = np.zeros_like(y)
+        radii = np.zeros(nt, np.float32)
+        L.agbnp3_get_tables(self.tag, _p(y), _p(y2), _p(radii))
+        return y.reshape(nt * nt, 512), y2.reshape(nt * nt, 512), radii
+
+    def internal(self, name):
+        out = np.zeros(self.n)
+        if load().agbnp3_get_internal(self.tag, name.encode(), _p(out)) != AGBNP_OK:
+            raise KeyError(name)
+        return out
+
+    def counts(self):
+        c = np.zeros(5, np.int64)
+        load().agbnp3_get_counts(self.tag, _p(c))
+        return dict(kept_overlaps=int(c[0]), evaluated_overlaps=int(c[1]), water_sites=int(c[2]),
+                    near_pairs=int(c[3]), max_row=int(c[4]))
+
+    def kernel_times(self):
+        ms = np.zeros(64, np.float32)
+        names = ctypes.create_string_buffer(4096)
+        k = load().agbnp3_kernel_times(self.tag, _p(ms), 64, names, 4096)
+        return dict(zip(names.value.decode().split(";"), ms[:k].tolist())) if k > 0 else {}
+
+    def stream(self):
+        return load().agbnp3_stream(self.tag)
+
+    def ener_device(self, x, y, z, sp, br, sa, dgbdr, dvwdr, decav, dehb, energies):
+        """All arguments are device pointers (ints)."""
+        return load().agbnp3_ener_device(self.tag, x, y, z, sp, br, sa, dgbdr, dvwdr, decav, dehb, energies)
+
+    def sync(self):
+        return load().agbnp3_sync(self.tag)

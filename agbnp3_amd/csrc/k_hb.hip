@@ -1,0 +1,430 @@
+// Hydrogen-bond (water-site) term.
+//
+// Placement: one thread per HB-active atom writes its 1-4 pseudo-atoms of
+// radius 1.4 A and the Jacobians dpos[p][i][j] = d pos_i / d x_{parent p, j}
+// for the eight geometries of the reference (agbnp3.c:2014-3761).  Which
+// atoms get which geometry is fixed by the topology and decided on the host
+// when the instance is created.
+//
+// Free volume: one wavefront per site.  Heavy atoms passing the FP32 test
+// d2 < ((R + 1.4) * 1.6)^2 are collected from the cell grid and sorted by
+// atom index; their Gaussian overlaps are evaluated lane-parallel, then one
+// lane walks them in index order exactly as the reference does
+// (agbnp3_ws_free_volumes_scalev_ps, kernels.c:2774-2934): subtract
+// spe * V' while the free volume exceeds 0.6 V_w.  By default the spe read is
+// the previous overlap partner's (the reference's stale index, first one
+// given by the previous site's last parent); AGBNP3_OPT_HB_FIXED uses the
+// partner's own.  E_HB = sum khb s(V_free / V_w; 0.65, 1).
+#include <float.h>
+
+#include "agb_common.h"
+#include "agb_device.h"
+
+namespace agb {
+
+// --------------------------------------------------------------- 3-vectors
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 ld3(const float4 &p) { return {p.x, p.y, p.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float comp(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__device__ __forceinline__ f3 row3(const float M[3][3], int i) { return {M[i][0], M[i][1], M[i][2]}; }
+
+// d u / d r of u = r/|r|: (I - u u^T)/|r|
+__device__ __forceinline__ void unit_der(f3 u, float invr, float M[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) M[i][j] = invr * ((i == j ? 1.f : 0.f) - comp(u, i) * comp(u, j));
+}
+__device__ __forceinline__ void mm3(const float A[3][3], const float B[3][3], float C[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) C[i][j] = A[i][0] * B[0][j] + A[i][1] * B[1][j] + A[i][2] * B[2][j];
+}
+__device__ __forceinline__ f3 unit(f3 v, float &inv) {
+  inv = 1.0f / sqrtf(dot(v, v));
+  return inv * v;
+}
+
+__device__ __forceinline__ void site_init(WSite &w, f3 pos, float khb, int type, int np, int p0, int p1, int p2,
+                                          int p3) {
+  w.pos[0] = pos.x; w.pos[1] = pos.y; w.pos[2] = pos.z;
+  w.khb = khb;
+  w.type = type;
+  w.nparents = np;
+  w.parent[0] = p0; w.parent[1] = p1; w.parent[2] = p2; w.parent[3] = p3;
+  for (int p = 0; p < 4; p++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) w.dpos[p][i][j] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_ws_place(Dev d) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= d.nwsdef) return;
+  const WsDef def = d.wsdef[t];
+  const float L = kHbLength;
+  const int A = def.atom;
+  const f3 xa = ld3(d.pos[A]);
+  const float khb = d.hbcorr_f[A];
+  WSite *W = d.ws + def.first;
+  switch (def.kind) {
+    case WS_POLARH: {     // agbnp3_place_wat_hydrogen (agbnp3.c:2098-2150)
+      const int D = def.p[1];
+      const f3 xd = ld3(d.pos[D]);
+      float inv;
+      const f3 u = unit(xa - xd, inv);
+      WSite w;
+      site_init(w, xd + L * u, khb, 1, 2, A, D, -1, -1);
+      const float c = L * inv;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          const float m = c * ((i == j ? 1.f : 0.f) - comp(u, i) * comp(u, j));
+          w.dpos[0][i][j] = m;
+          w.dpos[1][i][j] = (i == j ? 1.f : 0.f) - m;
+        }
+      W[0] = w;
+      break;
+    }
+    case WS_TRIGONAL1:
+    case WS_TRIGONAL_S: {   // agbnp3_place_wat_trigonal1 (agbnp3.c:2295-2366)
+      const int R = def.p[1], R1 = def.p[2], R2 = def.p[3];
+      const f3 xr = ld3(d.pos[R]);
+      const int type = def.kind == WS_TRIGONAL1 ? 11 : 30;
+      for (int s = 0; s < 2; s++) {
+        float inv;
+        const f3 u = unit(ld3(d.pos[s == 0 ? R1 : R2]) - xr, inv);
+        WSite w;
+        site_init(w, xa - L * u, khb, type, 4, A, R, R1, R2);
+        const float c = L * inv;
+        for (int i = 0; i < 3; i++) {
+          w.dpos[0][i][i] = 1.f;
+          for (int j = 0; j < 3; j++) {
+            const float m = c * ((i == j ? 1.f : 0.f) - comp(u, i) * comp(u, j));
+            w.dpos[1][i][j] = m;
+            w.dpos[2 + s][i][j] = -m;
+          }
+        }
+        W[s] = w;
+      }
+      if (def.kind == WS_TRIGONAL_S) {   // agbnp3_place_wat_trigonal_s (agbnp3.c:2590-2697)
+        const float cs = 0.5f, sn = 0.866025f;
+        const f3 dx0 = xa - xr, dx1 = ld3(d.pos[R1]) - xr;
+        float d0i, d2i;
+        const f3 uin = unit(dx0, d0i);
+        const f3 uout = unit(cross(dx1, dx0), d2i);
+        float Din[3][3], Dout[3][3], t1[3][3], t2[3][3];
+        unit_der(uin, d0i, Din);
+        unit_der(uout, d2i, Dout);
+        for (int i = 0; i < 3; i++) {
+          const f3 a = cross(dx1, row3(Dout, i)), b = cross(dx0, row3(Dout, i));
+          t1[i][0] = a.x; t1[i][1] = a.y; t1[i][2] = a.z;
+          t2[i][0] = b.x; t2[i][1] = b.y; t2[i][2] = b.z;
+        }
+        for (int s = 0; s < 2; s++) {
+          const float sg = s == 0 ? 1.f : -1.f;
+          WSite w;
+          site_init(w, xa + L * (cs * uin + (sg * sn) * uout), khb, 30, 4, A, R, R1, R2);
+          for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+              const float win = L * cs * Din[i][j];
+              const float w1 = -L * sn * t1[i][j], w2 = L * sn * t2[i][j];
+              w.dpos[0][i][j] = win + sg * w1 + (i == j ? 1.f : 0.f);
+              w.dpos[1][i][j] = -win - sg * (w1 + w2);
+              w.dpos[2][i][j] = sg * w2;
+            }
+          W[2 + s] = w;
+        }
+      }
+      break;
+    }
+    case WS_TRIGONAL_OOP: {   // agbnp3_place_wat_trigonal_oop (agbnp3.c:2815-2942)
+      const int R1 = def.p[1], R2 = def.p[2], R3 = def.p[3];
+      const float dd = L + 0.2f;
+      const f3 x1 = ld3(d.pos[R1]);
+      const f3 v1 = ld3(d.pos[R2]) - x1, v2 = ld3(d.pos[R3]) - x1;
+      const f3 nraw = cross(v2, v1);
+      const float nun = sqrtf(dot(nraw, nraw));
+      const f3 nu = (1.0f / nun) * nraw;
+      const float J1[3][3] = {{0.f, -v2.z, v2.y}, {v2.z, 0.f, -v2.x}, {-v2.y, v2.x, 0.f}};
+      const float J2[3][3] = {{0.f, v1.z, -v1.y}, {-v1.z, 0.f, v1.x}, {v1.y, -v1.x, 0.f}};
+      float P[3][3], M1[3][3], M2[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) P[i][j] = (i == j ? 1.f : 0.f) - comp(nu, i) * comp(nu, j);
+      mm3(P, J1, M1);
+      mm3(P, J2, M2);
+      const float dn = dd / nun;
+      for (int s = 0; s < 2; s++) {
+        const float sg = s == 0 ? 1.f : -1.f;
+        WSite w;
+        site_init(w, xa + (sg * dd) * nu, khb, 13, 4, A, R1, R2, R3);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            w.dpos[0][i][j] = i == j ? 1.f : 0.f;
+            w.dpos[2][i][j] = sg * dn * M1[i][j];
+            w.dpos[3][i][j] = sg * dn * M2[i][j];
+            w.dpos[1][i][j] = -(w.dpos[2][i][j] + w.dpos[3][i][j]);
+          }
+        W[s] = w;
+      }
+      break;
+    }
+    case WS_TRIGONAL2:
+    case WS_TETRAHEDRAL3: {   // agbnp3_place_wat_trigonal2 / _tetrahedral3 (agbnp3.c:3024-3090, 3476-3556)
+      const int nr = def.kind == WS_TRIGONAL2 ? 2 : 3;
+      f3 dk[3];
+      float inv[3];
+      f3 sum = {0.f, 0.f, 0.f};
+      for (int k = 0; k < nr; k++) {
+        dk[k] = unit(ld3(d.pos[def.p[1 + k]]) - xa, inv[k]);
+        sum = sum + dk[k];
+      }
+      float iw;
+      const f3 dw = unit(sum, iw);
+      WSite w;
+      site_init(w, xa - L * dw, khb, nr == 2 ? 12 : 23, nr + 1, A, def.p[1], def.p[2], nr == 3 ? def.p[3] : -1);
+      float Dw[3][3];
+      unit_der(dw, iw, Dw);
+      for (int i = 0; i < 3; i++) w.dpos[0][i][i] = 1.f;
+      for (int k = 0; k < nr; k++) {
+        float Dk[3][3], M[3][3];
+        unit_der(dk[k], inv[k], Dk);
+        mm3(Dw, Dk, M);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            w.dpos[1 + k][i][j] = -L * M[i][j];
+            w.dpos[0][i][j] += L * M[i][j];
+          }
+      }
+      W[0] = w;
+      break;
+    }
+    case WS_TETRAHEDRAL2: {   // agbnp3_place_wat_tetrahedral2 (agbnp3.c:3266-3390)
+      const float cs = -0.577350269f, sn = 0.816496581f;
+      const int R1 = def.p[1], R2 = def.p[2];
+      const f3 r1 = ld3(d.pos[R1]) - xa, r2 = ld3(d.pos[R2]) - xa;
+      float i1, i2, iin, iout;
+      const f3 d1 = unit(r1, i1), d2 = unit(r2, i2);
+      const f3 pin = unit(d1 + d2, iin);
+      const f3 out = unit(cross(r2, r1), iout);
+      float Dp[3][3], D1[3][3], D2[3][3], P1[3][3], P2[3][3], Do[3][3], t1[3][3], t2[3][3];
+      unit_der(pin, iin, Dp);
+      unit_der(d1, i1, D1);
+      unit_der(d2, i2, D2);
+      mm3(Dp, D1, P1);
+      mm3(Dp, D2, P2);
+      unit_der(out, iout, Do);
+      for (int i = 0; i < 3; i++) {
+        const f3 a = cross(row3(Do, i), r2), b = cross(r1, row3(Do, i));
+        t1[i][0] = a.x; t1[i][1] = a.y; t1[i][2] = a.z;
+        t2[i][0] = b.x; t2[i][1] = b.y; t2[i][2] = b.z;
+      }
+      for (int s = 0; s < 2; s++) {
+        const float sg = s == 0 ? 1.f : -1.f;
+        WSite w;
+        site_init(w, xa + L * (cs * pin + (sg * sn) * out), khb, 22, 3, A, R1, R2, -1);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            const float a1 = L * cs * P1[i][j], a2 = L * cs * P2[i][j];
+            const float b1 = L * sn * t1[i][j], b2 = L * sn * t2[i][j];
+            w.dpos[1][i][j] = a1 + sg * b1;
+            w.dpos[2][i][j] = a2 + sg * b2;
+            w.dpos[0][i][j] = -(a1 + a2) - sg * (b1 + b2) + (i == j ? 1.f : 0.f);
+          }
+        W[s] = w;
+      }
+      break;
+    }
+    case WS_TETRAHEDRAL1: {   // agbnp3_place_wat_tetrahedral1_one (agbnp3.c:3689-3761)
+      const int R = def.p[1];
+      const f3 xr = ld3(d.pos[R]);
+      for (int s = 0; s < 3; s++) {
+        const int Rk = def.p[2 + s];
+        float inv;
+        const f3 u = unit(ld3(d.pos[Rk]) - xr, inv);
+        WSite w;
+        site_init(w, xa - L * u, khb, 21, 3, A, R, Rk, -1);
+        const float c = L * inv;
+        for (int i = 0; i < 3; i++) {
+          w.dpos[0][i][i] = 1.f;
+          for (int j = 0; j < 3; j++) {
+            const float m = c * ((i == j ? 1.f : 0.f) - comp(u, i) * comp(u, j));
+            w.dpos[1][i][j] = m;
+            w.dpos[2][i][j] = -m;
+          }
+        }
+        W[s] = w;
+      }
+      break;
+    }
+  }
+  for (int s = 0; s < def.nsite; s++) W[s].stale0 = d.ws_stale0[def.first + s];
+}
+
+void launch_ws_place(const Dev &d, hipStream_t s) {
+  if (d.nwsdef == 0) return;
+  hipLaunchKernelGGL(k_ws_place, dim3((d.nwsdef + 255) / 256), dim3(256), 0, s, d);
+}
+
+// ------------------------------------------------------------ free volume
+constexpr int kWsWaves = 4;
+constexpr int kWsCap = 128;
+
+__global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
+  __shared__ int cj[kWsWaves][kWsCap];
+  __shared__ int cs[kWsWaves][kWsCap];        // sorted candidate atoms
+  __shared__ float cvp[kWsWaves][kWsCap];
+  __shared__ float cgw[kWsWaves][kWsCap][3];
+  __shared__ float cgi[kWsWaves][kWsCap][3];
+  __shared__ int ckeep[kWsWaves][kWsCap];
+  __shared__ double sfree[kWsWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int site = blockIdx.x * kWsWaves + w;
+  const bool active = site < d.nws;
+  WSite W;
+  if (active) W = d.ws[site];
+  const GridParams g = *d.grid;
+  const float aw = kKFC / (kHbRadius * kHbRadius);
+  int cnt = 0;
+  if (active) {
+    int cx, cy, cz;
+    cell_coords(g, W.pos[0], W.pos[1], W.pos[2], cx, cy, cz);
+    for (int dz = -1; dz <= 1; dz++) {
+      const int zz = cz + dz;
+      if (zz < 0 || zz >= g.dim[2]) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int yy = cy + dy;
+        if (yy < 0 || yy >= g.dim[1]) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int xx = cx + dx;
+          if (xx < 0 || xx >= g.dim[0]) continue;
+          const int c = (zz * g.dim[1] + yy) * g.dim[0] + xx;
+          const int beg = d.cell_start[c], end = d.cell_start[c + 1];
+          for (int base = beg; base < end; base += 64) {
+            const int k = base + lane;
+            bool pass = false;
+            int j = -1;
+            if (k < end) {
+              j = d.cell_atoms[k];
+              const float4 pj = d.pos[j];
+              const float ex = __fsub_rn(pj.x, W.pos[0]), ey = __fsub_rn(pj.y, W.pos[1]),
+                          ez = __fsub_rn(pj.z, W.pos[2]);
+              const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ex, ex), __fmul_rn(ey, ey)), __fmul_rn(ez, ez));
+              const float u = __fmul_rn(__fadd_rn(pj.w, kHbRadius), kNbOffset);
+              pass = d2 < __fmul_rn(u, u);
+            }
+            const unsigned long long m = __ballot(pass);
+            if (pass) {
+              const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
+              if (slot < kWsCap) cj[w][slot] = j;
+            }
+            cnt += __popcll(m);
+          }
+        }
+      }
+    }
+  }
+  if (cnt > kWsCap) {
+    if (lane == 0) atomicOr(d.status, ST_WS_OVERFLOW);
+    cnt = kWsCap;
+  }
+  __syncthreads();
+  // sort by atom index, evaluate overlaps lane-parallel
+  for (int e = lane; e < cnt; e += 64) {
+    const int je = cj[w][e];
+    int rank = 0;
+    for (int f = 0; f < cnt; f++) rank += cj[w][f] < je;
+    cs[w][rank] = je;
+  }
+  __syncthreads();
+  for (int e = lane; e < cnt; e += 64) {
+    const int k = cs[w][e];
+    const float4 pk = d.pos[k];
+    const float ak = d.galpha[k];
+    const float ex = pk.x - W.pos[0], ey = pk.y - W.pos[1], ez = pk.z - W.pos[2];
+    const float d2 = ex * ex + ey * ey + ez * ez;
+    const float a12 = aw + ak, dinv = 1.0f / a12;
+    const float p12 = kPFC * kPFC * __expf(-aw * ak * d2 * dinv);
+    const float u = kPiF * dinv;
+    const float V = p12 * u * sqrtf(u);
+    float s, sp, spp;
+    vol_switch(V, s, sp, spp);
+    const float fp = s + V * sp;
+    const float c[3] = {dinv * (aw * W.pos[0] + ak * pk.x), dinv * (aw * W.pos[1] + ak * pk.y),
+                        dinv * (aw * W.pos[2] + ak * pk.z)};
+    const float xk[3] = {pk.x, pk.y, pk.z};
+    cvp[w][e] = s * V;
+    for (int q = 0; q < 3; q++) {
+      cgw[w][e][q] = -2.f * aw * V * fp * (W.pos[q] - c[q]);
+      cgi[w][e][q] = -2.f * ak * V * fp * (xk[q] - c[q]);
+    }
+  }
+  __syncthreads();
+  const double vw = 4.0 / 3.0 * kPi * (double)kHbRadius * (double)kHbRadius * (double)kHbRadius;
+  if (lane == 0 && active) {
+    // the reference's sequential walk with early stop (kernels.c:2818-2860)
+    const bool fixed = (d.options & 1) != 0;
+    double freev = vw;
+    const double cutoff = (double)kHbSwA0 * vw;
+    int stale = W.stale0;
+    for (int e = 0; e < cnt; e++) ckeep[w][e] = 0;
+    for (int e = 0; e < cnt; e++) {
+      if (!(freev > cutoff)) break;
+      const float vp = cvp[w][e];
+      if (vp > 0.f) {
+        const int k = cs[w][e];
+        const int idx = fixed ? k : stale;
+        const double sp = idx < d.n ? d.spe[idx] : 0.0;
+        freev -= sp * (double)vp;
+        stale = k;
+        ckeep[w][e] = 1;
+      }
+    }
+    sfree[w] = freev;
+  }
+  __syncthreads();
+  if (!active) return;
+  const double spw = sfree[w] / vw;
+  double fp = 0.0, sw;
+  {
+    const double xa = kHbSwA, xb = kHbSwB;
+    if (spw > xb) sw = 1.0;
+    else if (spw < xa) sw = 0.0;
+    else {
+      const double dd = 1.0 / (xb - xa), uu = (spw - xa) * dd, u2 = uu * uu;
+      sw = u2 * uu * (10.0 - 15.0 * uu + 6.0 * u2);
+      fp = dd * 30.0 * u2 * (1.0 - 2.0 * uu + u2);
+    }
+  }
+  const double dhw = (double)W.khb * fp / vw;
+  if (lane == 0) d.ws_part[site] = (double)W.khb * sw;
+  double jw[3] = {0.0, 0.0, 0.0};
+  for (int e = lane; e < cnt; e += 64) {
+    if (!ckeep[w][e]) continue;
+    const int k = cs[w][e];
+    const double spk = d.spe[k];
+    for (int q = 0; q < 3; q++) {
+      fix_add(&d.acc_hb[3 * (size_t)k + q], -spk * dhw * (double)cgi[w][e][q]);
+      jw[q] += -dhw * spk * (double)cgw[w][e][q];
+    }
+    fix_add(&d.acc_derh[k], dhw / d.vols[k] * (double)cvp[w][e]);
+  }
+  for (int q = 0; q < 3; q++) jw[q] = wave_sum(jw[q]);
+  // project the site force onto its parents: dpos^T jw (agbnp3_rtvec)
+  if (lane < 3 * W.nparents) {
+    const int p = lane / 3, j = lane - 3 * p;
+    const double g = jw[0] * W.dpos[p][0][j] + jw[1] * W.dpos[p][1][j] + jw[2] * W.dpos[p][2][j];
+    fix_add(&d.acc_hb[3 * (size_t)W.parent[p] + j], g);
+  }
+}
+
+void launch_ws_free(const Dev &d, hipStream_t s) {
+  if (d.nws == 0) return;
+  hipLaunchKernelGGL(k_ws_free, dim3((d.nws + kWsWaves - 1) / kWsWaves), dim3(64 * kWsWaves), 0, s, d);
+}
+
+}  // namespace agb

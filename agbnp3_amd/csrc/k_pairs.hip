@@ -1,0 +1,248 @@
+// All-pairs kernels: pairwise descreening (inverse Born radii), GB pair
+// energy with A_i and the constant-B gradient, and the chain rule at
+// constant self volumes fused with U_i / V_i.
+//
+// Layout: a block owns 256 rows (one per thread) and a chunk of columns
+// (gridDim.y chunks); columns are staged through LDS 256 at a time and
+// broadcast to the lanes.  Each row sums a tile in FP32 and tiles in FP64;
+// per-chunk row sums go to [chunk][row] partials reduced in a fixed order by
+// the epilogue kernels, so no atomics and bitwise-reproducible results.
+// For the spline kernels rows are grouped by radius type (row_perm, 256-row
+// blocks of one type), so a block stages only the tables its rows can use.
+#include "agb_common.h"
+#include "agb_device.h"
+
+namespace agb {
+
+constexpr int kRows = 256;
+constexpr int kTile = 256;
+
+// ------------------------------------------------------------ descreening
+// beta_i = 1/(R_i - 0.5) - (1/4pi) sum_{j heavy, j != i} sp_j Q4(d_ij; i <- j)
+// Q4 = f(d/R_j)/R_j from table [rtype_i][rtype_j]
+// (reference agbnp3_inverse_born_radii_nolist_soa, kernels.c:500-678)
+__global__ __launch_bounds__(kRows) void k_descreen(Dev d) {
+  extern __shared__ float2 dyn_tab[];     // [ntypes][512]: tables [t_row][*]
+  float2 (*tab)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab);
+  __shared__ float4 cpos[kTile];     // x, y, z, 1/R
+  __shared__ float2 cw[kTile];       // sp/R, rtype (as float bits)
+  const int b = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
+  const int t = d.block_type[b];
+  const int nt = d.ntypes;
+  for (int k = threadIdx.x; k < nt * kTblN; k += kRows) {
+    const int tj = k / kTblN, node = k - tj * kTblN;
+    tab[tj][node] = d.tbl[(size_t)(t * nt + tj) * kTblN + node];
+  }
+  const int i = d.row_perm[b * kRows + threadIdx.x];
+  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i >= 0) pi = d.pos[i];
+  const int per = (d.nh + nchunk - 1) / nchunk;
+  const int c0 = chunk * per, c1 = min(d.nh, c0 + per);
+  const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
+  double acc = 0.0;
+  for (int base = c0; base < c1; base += kTile) {
+    __syncthreads();
+    const int j = base + threadIdx.x;
+    if (j < c1) {
+      const float4 pj = d.pos[j];
+      const float iR = 1.0f / pj.w;
+      cpos[threadIdx.x] = make_float4(pj.x, pj.y, pj.z, iR);
+      cw[threadIdx.x] = make_float2(d.sp_f[j] * iR, __int_as_float(d.rtype[j]));
+    }
+    __syncthreads();
+    const int nc = min(kTile, c1 - base);
+    float tacc = 0.f;
+    if (i >= 0) {
+      for (int c = 0; c < nc; c++) {
+        const float4 pj = cpos[c];
+        const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+        const float2 w = cw[c];
+        float q, dq;
+        q4_lookup<true, false>(tab[__float_as_int(w.y)], dist * pj.w, dx_, dxinv, dp1, dp2, 1.f, q, dq);
+        tacc += (base + c != i) ? q * w.x : 0.f;
+      }
+    }
+    acc += (double)tacc;
+  }
+  if (i >= 0) d.beta_part[(size_t)chunk * d.n + i] = acc;
+}
+
+void launch_descreen(const Dev &d, hipStream_t s) {
+  const size_t lds = (size_t)d.ntypes * kTblN * sizeof(float2);
+  hipLaunchKernelGGL(k_descreen, dim3(d.nrow_blocks, d.nchunk_desc), dim3(kRows), lds, s, d);
+}
+
+// ------------------------------------------------------------- GB pairs
+// per row i, all j != i (both orders, so each pair is counted twice):
+//   e    = q_i q_j / f,  f = sqrt(d2 + B_i B_j exp(-d2 / 4 B_i B_j))
+//   g    = q_i q_j (1 - e/4) / f^3 (x_j - x_i)      (x 2 df later)
+//   A_i  = q_i q_j (B_i B_j + d2/4) exp(.) / f^3
+// (reference agbnp3_gb_energy_inner_nolist_soa, kernels.c:66-151)
+__global__ __launch_bounds__(kRows) void k_gbpair(Dev d) {
+  __shared__ float4 cpos[kTile];   // x, y, z, q
+  __shared__ float cb[kTile];      // B
+  const int i = blockIdx.x * kRows + threadIdx.x;
+  const int chunk = blockIdx.y, nchunk = gridDim.y;
+  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+  float qi = 0.f, bi = 1.f;
+  if (i < d.n) {
+    pi = d.pos[i];
+    qi = d.q[i];
+    bi = d.br_f[i];
+  }
+  const int per = (d.n + nchunk - 1) / nchunk;
+  const int c0 = chunk * per, c1 = min(d.n, c0 + per);
+  double A = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, e = 0.0;
+  for (int base = c0; base < c1; base += kTile) {
+    __syncthreads();
+    const int j = base + threadIdx.x;
+    if (j < c1) {
+      const float4 pj = d.pos[j];
+      cpos[threadIdx.x] = make_float4(pj.x, pj.y, pj.z, d.q[j]);
+      cb[threadIdx.x] = d.br_f[j];
+    }
+    __syncthreads();
+    const int nc = min(kTile, c1 - base);
+    float tA = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, te = 0.f;
+    for (int c = 0; c < nc; c++) {
+      const float4 pj = cpos[c];
+      const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+      const float d2 = ex * ex + ey * ey + ez * ez;
+      const bool valid = (base + c != i) && (d2 > 0.f);
+      const float qq = valid ? qi * pj.w : 0.f;
+      const float bb = bi * cb[c];
+      const float et = __expf(-0.25f * d2 * __builtin_amdgcn_rcpf(bb));
+      const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
+      const float fgb3 = fgb * fgb * fgb;
+      te += qq * fgb;
+      const float mw = qq * (1.f - 0.25f * et) * fgb3;
+      tx += mw * ex;
+      ty += mw * ey;
+      tz += mw * ez;
+      tA += qq * (bb + 0.25f * d2) * et * fgb3;
+    }
+    A += tA; gx += tx; gy += ty; gz += tz; e += te;
+  }
+  if (i < d.n) {
+    double *p = d.gb_part + (size_t)chunk * 5 * d.n;
+    p[i] = A;
+    p[(size_t)d.n + i] = gx;
+    p[2 * (size_t)d.n + i] = gy;
+    p[3 * (size_t)d.n + i] = gz;
+    p[4 * (size_t)d.n + i] = e;
+  }
+}
+
+void launch_gbpair(const Dev &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_gbpair, dim3((d.n + kRows - 1) / kRows, d.nchunk_gb), dim3(kRows), 0, s, d);
+}
+
+// -------------------------------------------- chain rule at constant sp
+// Row k collects both directed descreening pairs with every atom j:
+//   k <- j (j heavy):  q2ab_k dQ4(k<-j) sp_j      (table [t_k][t_j], R_j)
+//   j <- k (k heavy):  q2ab_j dQ4(j<-k) sp_k      (table [t_j][t_k], R_k)
+// h = df/(4pi) (sum)/d along (x_j - x_k) for the GB gradient, the same with
+// abrw and 1/(4pi) for vdW; U_k = sum_j q2ab_j Q4(j<-k), V_k likewise with abrw.
+// (reference agbnp3_gb_ders_constvp_nolist_ps kernels.c:681-805 and
+//  agbnp3_gb_deruv_nolist_ps kernels.c:809-904, without the q4cache)
+__global__ __launch_bounds__(kRows) void k_chain(Dev d) {
+  extern __shared__ float2 dyn_tab[];              // [2][ntypes][512]
+  float2 (*tab_kj)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab);                       // [t_k][*]
+  float2 (*tab_jk)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab + (size_t)d.ntypes * kTblN);  // [*][t_k]
+  __shared__ float4 cpos[kTile];                   // x, y, z, 1/R
+  __shared__ float4 cpar[kTile];                   // sp, q2ab, abrw, rtype bits
+  const int b = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
+  const int t = d.block_type[b];
+  const int nt = d.ntypes;
+  for (int k = threadIdx.x; k < nt * kTblN; k += kRows) {
+    const int tj = k / kTblN, node = k - tj * kTblN;
+    tab_kj[tj][node] = d.tbl[(size_t)(t * nt + tj) * kTblN + node];
+    tab_jk[tj][node] = d.tbl[(size_t)(tj * nt + t) * kTblN + node];
+  }
+  const int kr = d.row_perm[b * kRows + threadIdx.x];
+  float4 pk = make_float4(0.f, 0.f, 0.f, 1.f);
+  float spk = 0.f, q2k = 0.f, abk = 0.f;
+  bool kheavy = false;
+  if (kr >= 0) {
+    pk = d.pos[kr];
+    q2k = d.q2ab_f[kr];
+    abk = d.abrw_f[kr];
+    kheavy = kr < d.nh;
+    spk = kheavy ? d.sp_f[kr] : 0.f;
+  }
+  const float iRk = 1.0f / pk.w;
+  // hydrogen rows only pair with heavy columns; a block's rows share a type,
+  // but heavy and hydrogen rows of equal radius may share a block
+  const int ncol = d.n;
+  const int per = (ncol + nchunk - 1) / nchunk;
+  const int c0 = chunk * per, c1 = min(ncol, c0 + per);
+  const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
+  const float sgn2 = (d.options & 2) ? -1.f : 1.f;
+  double gb[3] = {0, 0, 0}, vw[3] = {0, 0, 0}, U = 0.0, V = 0.0;
+  for (int base = c0; base < c1; base += kTile) {
+    __syncthreads();
+    const int j = base + threadIdx.x;
+    if (j < c1) {
+      const float4 pj = d.pos[j];
+      cpos[threadIdx.x] = make_float4(pj.x, pj.y, pj.z, 1.0f / pj.w);
+      cpar[threadIdx.x] = make_float4(j < d.nh ? d.sp_f[j] : 0.f, d.q2ab_f[j], d.abrw_f[j],
+                                      __int_as_float(d.rtype[j]));
+    }
+    __syncthreads();
+    const int nc = min(kTile, c1 - base);
+    float tg[3] = {0.f, 0.f, 0.f}, tv[3] = {0.f, 0.f, 0.f}, tU = 0.f, tV = 0.f;
+    if (kr >= 0) {
+      for (int c = 0; c < nc; c++) {
+        const int j = base + c;
+        const bool jheavy = j < d.nh;
+        if (j == kr || (!kheavy && !jheavy)) continue;
+        const float4 pj = cpos[c];
+        const float4 pp = cpar[c];
+        const float ex = pj.x - pk.x, ey = pj.y - pk.y, ez = pj.z - pk.z;
+        const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+        const int tj = __float_as_int(pp.w);
+        float hgb = 0.f, hvw = 0.f;
+        if (jheavy) {
+          float q, dq;
+          q4_lookup<false, true>(tab_kj[tj], dist * pj.w, dx_, dxinv, dp1, dp2, sgn2, q, dq);
+          dq *= pj.w * pj.w;
+          hgb += q2k * pp.x * dq;
+          hvw += abk * pp.x * dq;
+        }
+        if (kheavy) {
+          float q, dq;
+          q4_lookup<true, true>(tab_jk[tj], dist * iRk, dx_, dxinv, dp1, dp2, sgn2, q, dq);
+          q *= iRk;
+          dq *= iRk * iRk;
+          hgb += pp.y * spk * dq;
+          hvw += pp.z * spk * dq;
+          tU += pp.y * q;
+          tV += pp.z * q;
+        }
+        const float id = 1.0f / dist;
+        hgb *= id;
+        hvw *= id;
+        tg[0] += hgb * ex; tg[1] += hgb * ey; tg[2] += hgb * ez;
+        tv[0] += hvw * ex; tv[1] += hvw * ey; tv[2] += hvw * ez;
+      }
+    }
+    for (int q = 0; q < 3; q++) { gb[q] += tg[q]; vw[q] += tv[q]; }
+    U += tU;
+    V += tV;
+  }
+  if (kr >= 0) {
+    double *p = d.ch_part + (size_t)chunk * 8 * d.n;
+    const size_t n = d.n;
+    p[kr] = gb[0]; p[n + kr] = gb[1]; p[2 * n + kr] = gb[2];
+    p[3 * n + kr] = vw[0]; p[4 * n + kr] = vw[1]; p[5 * n + kr] = vw[2];
+    p[6 * n + kr] = U; p[7 * n + kr] = V;
+  }
+}
+
+void launch_chain(const Dev &d, hipStream_t s) {
+  const size_t lds = 2 * (size_t)d.ntypes * kTblN * sizeof(float2);
+  hipLaunchKernelGGL(k_chain, dim3(d.nrow_blocks, d.nchunk_ch), dim3(kRows), lds, s, d);
+}
+
+}  // namespace agb

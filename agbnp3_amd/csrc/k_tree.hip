@@ -1,0 +1,331 @@
+// Gaussian-overlap tree: self volumes and surface areas (pass A) and their
+// coordinate derivatives (pass B).
+//
+// Tree (reference agbnp3_self_volumes_rooti, agbnp3_cpu_simd_kernels.c:1064-1598):
+// the order-2 overlaps of root i are (i, j) for j in i's neighbour list, in
+// list order; the children of an overlap are formed with the last atom of
+// each later sibling in its group; an overlap is kept when its switched
+// volume exceeds FLT_MIN; orders 2..5.  Its inclusion-exclusion
+// coefficient is (-1)^(n+1)/n for the volume and (-1)^(n+1) for the surface.
+//
+// GPU mapping: one wavefront per root.  Its neighbour row ("slots": 0 = the
+// root, 1..m = list entries) is staged in LDS; the tree is walked depth-first
+// over sibling groups, each group evaluated lane-parallel and compacted in
+// order with ballots.  Every overlap of the root involves only slot atoms,
+// so contributions are reduced per slot in LDS and scattered to atoms once
+// per root with order-independent fixed-point atomics.  Contributions to the
+// atoms on the current path are shared by all lanes: they are summed in lane
+// registers and reduced across the wave when the path advances.
+//
+// Pass B re-walks the identical tree (same FP32 arithmetic, same pruning)
+// instead of storing it, and applies in one place the derivatives of
+// agbnp3_der_vp_rooti (kernels.c:1601-1956: self-volume changes through
+// U_i, V_i, H_i) and agbnp3_cavity_dersgb_rooti (kernels.c:1959-2405:
+// surface-area changes through gamma', dervs, derus).
+#include <float.h>
+
+#include "agb_common.h"
+#include "agb_device.h"
+
+namespace agb {
+
+template <int CAP, bool PB>
+struct TreeLds {
+  int idx[CAP + 1];
+  float a[CAP + 1], x[CAP + 1], y[CAP + 1], z[CAP + 1], R[CAP + 1];
+  // pass B per-slot coefficients: deru, derv, derh, gamma', dervs, derus
+  float u[PB ? CAP + 1 : 1], v[PB ? CAP + 1 : 1], h[PB ? CAP + 1 : 1];
+  float wc[PB ? CAP + 1 : 1], wv[PB ? CAP + 1 : 1], wg[PB ? CAP + 1 : 1];
+  // sibling groups of depths 0..2 (overlap Gaussian + last slot)
+  float ga[3][CAP], gp[3][CAP], gx[3][CAP], gy[3][CAP], gz[3][CAP];
+  int gs[3][CAP];
+  // per-slot accumulators: pass A (volume, surface) FP64, pass B 12 FP32
+  double accA[PB ? 1 : 2][CAP + 1];
+  float accB[PB ? TG_N : 1][CAP + 1];
+};
+
+// coefficients of an overlap of order n
+__device__ __forceinline__ float ovl_sign(int n) { return (n & 1) ? 1.0f : -1.0f; }  // (-1)^(n+1)
+
+struct Gauss {
+  float a, p, x, y, z;
+};
+
+// FP32 Gaussian overlap of `par` with atom (ab, xb) (kernels.c:2418-2477)
+struct OvlOut {
+  float V, Vp, fp, fpp, a, p, cx, cy, cz, dinv;
+};
+__device__ __forceinline__ OvlOut overlap(const Gauss &par, float ab, float xb, float yb, float zb) {
+  OvlOut o;
+  const float dx = xb - par.x, dy = yb - par.y, dz = zb - par.z;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float a12 = par.a + ab;
+  const float dinv = 1.0f / a12;
+  const float p12 = par.p * kPFC * __expf(-par.a * ab * d2 * dinv);
+  const float u = kPiF * dinv;
+  o.V = p12 * u * sqrtf(u);
+  float s, sp, spp;
+  vol_switch(o.V, s, sp, spp);
+  o.Vp = s * o.V;
+  o.fp = s + o.V * sp;
+  o.fpp = 2.f * sp + o.V * spp;
+  o.a = a12;
+  o.p = p12;
+  o.cx = dinv * (par.a * par.x + ab * xb);
+  o.cy = dinv * (par.a * par.y + ab * yb);
+  o.cz = dinv * (par.a * par.z + ab * zb);
+  o.dinv = dinv;
+  return o;
+}
+
+// pass-A contribution of one overlap to member (a, x, R): surface term
+__device__ __forceinline__ float sa_term(const OvlOut &o, float sign, float a, float x, float y, float z, float R) {
+  const float ex = x - o.cx, ey = y - o.cy, ez = z - o.cz;
+  const float t = ex * ex + ey * ey + ez * ez;
+  return sign * o.fp * a * o.V * (3.f * o.dinv + 2.f * t) / R;
+}
+
+// Evaluate the children of one tree node: lanes take candidate slots, keep
+// overlaps above FLT_MIN, add their contributions, and (if STORE) compact
+// them into group level L.  N = L + 2 is the order of the children; the
+// members are the path slots path[0..L] and the candidate.
+template <int L, int CAP, bool PB, bool STORE>
+__device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &par, const int *path,
+                                             int ncand, const int *cand_slots_lds, int cand_base,
+                                             double (&pa)[4][2], float (&pb)[4][TG_N], float df,
+                                             unsigned &nkept, unsigned &neval) {
+  constexpr int N = L + 2;
+  const int lane = threadIdx.x & 63;
+  const float sign = ovl_sign(N);
+  const float cn = sign / (float)N;
+  int cnt = 0;
+  // path member data (wave-uniform broadcast reads)
+  float ma[N - 1], mx[N - 1], my[N - 1], mz[N - 1], mR[N - 1];
+#pragma unroll
+  for (int k = 0; k < N - 1; k++) {
+    const int s = path[k];
+    ma[k] = S.a[s]; mx[k] = S.x[s]; my[k] = S.y[s]; mz[k] = S.z[s]; mR[k] = S.R[s];
+  }
+  float pU = 0.f, pV = 0.f, pH = 0.f;
+  float mwc[N - 1], mwv[N - 1], mwg[N - 1];
+  if (PB) {
+#pragma unroll
+    for (int k = 0; k < N - 1; k++) {
+      const int s = path[k];
+      pU += S.u[s]; pV += S.v[s]; pH += S.h[s];
+      mwc[k] = sign * S.wc[s]; mwv[k] = sign * S.wv[s]; mwg[k] = sign * S.wg[s];
+    }
+  }
+  for (int base = 0; base < ncand; base += 64) {
+    const int c = base + lane;
+    bool keep = false;
+    int sb = 0;
+    OvlOut o;
+    if (c < ncand) {
+      sb = cand_slots_lds ? cand_slots_lds[cand_base + c] : cand_base + c;
+      o = overlap(par, S.a[sb], S.x[sb], S.y[sb], S.z[sb]);
+      keep = o.Vp > FLT_MIN;
+    }
+    neval += (unsigned)__popcll(__ballot(c < ncand));
+    const unsigned long long bal = __ballot(keep);
+    if (keep) {
+      const float ab = S.a[sb], xb = S.x[sb], yb = S.y[sb], zb = S.z[sb], Rb = S.R[sb];
+      if (!PB) {
+        const double wv = (double)cn * (double)o.Vp;
+#pragma unroll
+        for (int k = 0; k < N - 1; k++) {
+          pa[k][0] += wv;
+          pa[k][1] += (double)sa_term(o, sign, ma[k], mx[k], my[k], mz[k], mR[k]);
+        }
+        S.accA[0][sb] += wv;
+        S.accA[1][sb] += (double)sa_term(o, sign, ab, xb, yb, zb, Rb);
+      } else {
+        // members: path[0..N-2] then the candidate (index N-1)
+        float A_[N], X_[N], Y_[N], Z_[N], R_[N], WC[N], WV[N], WG[N];
+#pragma unroll
+        for (int k = 0; k < N - 1; k++) {
+          A_[k] = ma[k]; X_[k] = mx[k]; Y_[k] = my[k]; Z_[k] = mz[k]; R_[k] = mR[k];
+          WC[k] = mwc[k]; WV[k] = mwv[k]; WG[k] = mwg[k];
+        }
+        A_[N - 1] = ab; X_[N - 1] = xb; Y_[N - 1] = yb; Z_[N - 1] = zb; R_[N - 1] = Rb;
+        WC[N - 1] = sign * S.wc[sb]; WV[N - 1] = sign * S.wv[sb]; WG[N - 1] = sign * S.wg[sb];
+        const float Us = pU + S.u[sb], Vs = pV + S.v[sb], Hs = pH + S.h[sb];
+        const float fourpi1 = 1.0f / (4.0f * kPiF);
+        const float Uc = -df * fourpi1 * cn * Us * o.fp;
+        const float Vc = -fourpi1 * cn * Vs * o.fp;
+        const float Hc = -cn * Hs * o.fp;
+        const float h3 = o.fpp + o.fp / o.V, h4 = 2.f * o.fp * o.dinv;
+        float gdr[N][3], gdR[N];
+        float S1c = 0.f, S1v = 0.f, S1g = 0.f;
+        float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          const float ex = X_[k] - o.cx, ey = Y_[k] - o.cy, ez = Z_[k] - o.cz;
+          const float av = A_[k] * o.V;
+          gdr[k][0] = -2.f * av * ex;
+          gdr[k][1] = -2.f * av * ey;
+          gdr[k][2] = -2.f * av * ez;
+          gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) / R_[k];
+          S1c += WC[k] * gdR[k]; S1v += WV[k] * gdR[k]; S1g += WG[k] * gdR[k];
+          const float iR = 1.0f / R_[k];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            S2c[q] += WC[k] * gdr[k][q] * iR;
+            S2v[q] += WV[k] * gdr[k][q] * iR;
+            S2g[q] += WG[k] * gdr[k][q] * iR;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+          const float iR = 1.0f / R_[k];
+          const float t = -2.f * o.fp * (1.f - A_[k] * o.dinv) * iR;
+          const float bc = h3 * S1c + WC[k] * t, bv = h3 * S1v + WV[k] * t, bg = h3 * S1g + WG[k] * t;
+          const float ha = h4 * A_[k];
+          float out[TG_N];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            const float g = gdr[k][q];
+            out[TG_GB + q] = g * bg + ha * (S2g[q] - WG[k] * g * iR) + Uc * g;
+            out[TG_VW + q] = g * bv + ha * (S2v[q] - WV[k] * g * iR) + Vc * g;
+            out[TG_CAV + q] = g * bc + ha * (S2c[q] - WC[k] * g * iR);
+            out[TG_HB + q] = Hc * g;
+          }
+          if (k < N - 1) {
+#pragma unroll
+            for (int v = 0; v < TG_N; v++) pb[k][v] += out[v];
+          } else {
+#pragma unroll
+            for (int v = 0; v < TG_N; v++) S.accB[v][sb] += out[v];
+          }
+        }
+      }
+      if (STORE) {
+        const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
+        S.ga[L][pos] = o.a; S.gp[L][pos] = o.p;
+        S.gx[L][pos] = o.cx; S.gy[L][pos] = o.cy; S.gz[L][pos] = o.cz;
+        S.gs[L][pos] = sb;
+      }
+    }
+    cnt += __popcll(bal);
+  }
+  nkept += (unsigned)cnt;
+  __syncthreads();
+  return cnt;
+}
+
+// reduce the lane accumulators of path depth k into the LDS slot `slot`
+template <int CAP, bool PB>
+__device__ __forceinline__ void flush_path(TreeLds<CAP, PB> &S, int k, int slot, double (&pa)[4][2],
+                                           float (&pb)[4][TG_N]) {
+  const int lane = threadIdx.x & 63;
+  if (!PB) {
+    const double v0 = wave_sum(pa[k][0]), v1 = wave_sum(pa[k][1]);
+    if (lane == 0) { S.accA[0][slot] += v0; S.accA[1][slot] += v1; }
+    pa[k][0] = pa[k][1] = 0.0;
+  } else {
+#pragma unroll
+    for (int v = 0; v < TG_N; v++) {
+      const float r = wave_sum(pb[k][v]);
+      if (lane == 0) S.accB[v][slot] += r;
+      pb[k][v] = 0.f;
+    }
+  }
+  __syncthreads();
+}
+
+template <int CAP, bool PB>
+__global__ __launch_bounds__(64) void k_tree(Dev d) {
+  __shared__ TreeLds<CAP, PB> S;
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x;
+  const int m = d.nne[i];
+  // stage the slot atoms
+  for (int s = lane; s <= m; s += 64) {
+    const int at = s == 0 ? i : d.nbr[(size_t)i * d.cap + s - 1];
+    const float4 p = d.pos[at];
+    S.idx[s] = at;
+    S.a[s] = d.galpha[at];
+    S.x[s] = p.x; S.y[s] = p.y; S.z[s] = p.z; S.R[s] = p.w;
+    if (PB) {
+      S.u[s] = d.deru_f[at]; S.v[s] = d.derv_f[at]; S.h[s] = d.derh_f[at];
+      S.wc[s] = d.gammap_f[at]; S.wv[s] = d.dervs_f[at]; S.wg[s] = d.derus_f[at];
+#pragma unroll
+      for (int v = 0; v < TG_N; v++) S.accB[v][s] = 0.f;
+    } else {
+      S.accA[0][s] = 0.0;
+      S.accA[1][s] = 0.0;
+    }
+  }
+  __syncthreads();
+  double pa[4][2];
+  float pb[4][TG_N];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    pa[k][0] = pa[k][1] = 0.0;
+#pragma unroll
+    for (int v = 0; v < TG_N; v++) pb[k][v] = 0.f;
+  }
+  unsigned nkept = 0, neval = 0;
+  const float df = d.eps_df;
+  int path[4];
+  path[0] = 0;
+  const Gauss root = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
+  // order 2: (i, j) for every list entry
+  const int n0 = eval_children<0, CAP, PB, true>(S, root, path, m, nullptr, 1, pa, pb, df, nkept, neval);
+  for (int i0 = 0; i0 < n0; i0++) {
+    path[1] = S.gs[0][i0];
+    if (i0 + 1 < n0) {
+      const Gauss g0 = {S.ga[0][i0], S.gp[0][i0], S.gx[0][i0], S.gy[0][i0], S.gz[0][i0]};
+      const int n1 = eval_children<1, CAP, PB, true>(S, g0, path, n0 - i0 - 1, &S.gs[0][0], i0 + 1, pa, pb, df,
+                                                     nkept, neval);
+      for (int i1 = 0; i1 < n1; i1++) {
+        path[2] = S.gs[1][i1];
+        if (i1 + 1 < n1) {
+          const Gauss g1 = {S.ga[1][i1], S.gp[1][i1], S.gx[1][i1], S.gy[1][i1], S.gz[1][i1]};
+          const int n2 = eval_children<2, CAP, PB, true>(S, g1, path, n1 - i1 - 1, &S.gs[1][0], i1 + 1, pa, pb,
+                                                         df, nkept, neval);
+          for (int i2 = 0; i2 < n2; i2++) {
+            path[3] = S.gs[2][i2];
+            if (i2 + 1 < n2) {
+              const Gauss g2 = {S.ga[2][i2], S.gp[2][i2], S.gx[2][i2], S.gy[2][i2], S.gz[2][i2]};
+              eval_children<3, CAP, PB, false>(S, g2, path, n2 - i2 - 1, &S.gs[2][0], i2 + 1, pa, pb, df, nkept,
+                                               neval);
+            }
+            flush_path(S, 3, path[3], pa, pb);
+          }
+        }
+        flush_path(S, 2, path[2], pa, pb);
+      }
+    }
+    flush_path(S, 1, path[1], pa, pb);
+  }
+  flush_path(S, 0, 0, pa, pb);
+  // scatter the per-slot sums
+  for (int s = lane; s <= m; s += 64) {
+    const int at = S.idx[s];
+    if (!PB) {
+      fix_add(&d.acc_vol[at], S.accA[0][s]);
+      fix_add(&d.acc_sa[at], S.accA[1][s]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], (double)S.accB[v][s]);
+    }
+  }
+  if (!PB && lane == 0) {
+    atomicAdd(&d.counters[0], (unsigned long long)nkept);
+    atomicAdd(&d.counters[1], (unsigned long long)neval);
+  }
+}
+
+void launch_tree(const Dev &d, int pass, hipStream_t s) {
+  if (d.nh == 0) return;
+  if (d.cap <= 128) {
+    if (pass == 0) hipLaunchKernelGGL((k_tree<128, false>), dim3(d.nh), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL((k_tree<128, true>), dim3(d.nh), dim3(64), 0, s, d);
+  } else {
+    if (pass == 0) hipLaunchKernelGGL((k_tree<256, false>), dim3(d.nh), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL((k_tree<256, true>), dim3(d.nh), dim3(64), 0, s, d);
+  }
+}
+
+}  // namespace agb

@@ -1,0 +1,165 @@
+"""Throughput of AGBNP3 energy + gradient on one MI355X.
+
+Workload (one "step"): one agbnp3_ener evaluation -- neighbour-list rebuild,
+self volumes and surface areas, Born radii, GB energy, vdW, cavity and
+hydrogen-bond terms with all gradients -- of the 25,000-atom config-1
+synthetic solute (seed 20260025), inputs and outputs resident in HBM
+(agbnp3_ener_device).  value = atoms x steps / wall seconds.
+
+Also reported: the roofline position of the dominant kernel (algorithmic
+FP32 flops of its pair work / its average duration from HIP events recorded
+on the instance stream during the timed steps) and the reference CPU path
+(the reference library built from its own sources, OpenMP) timed on this
+host on a bounded sample of the same workload.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "E+grad atom-evals/s: plain vs batched+surrogate GPU at ≤0.1% ΔG_solv err"
+PEAK_FP32 = 157.3e12            # MI355X dense FP32 vector (packed FMA), FLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(n, nh):
+    """Per-call pair flops counted once per unordered pair from the reference
+    source (agbnp3_cpu_simd_kernels.c:103-141, 560-673, 718-801, 837-858)."""
+    nH = n - nh
+    hh = nh * (nh - 1) / 2.0
+    return {
+        "descreen": 82.0 * hh + 45.0 * nh * nH,
+        "gbpair": 40.0 * n * (n - 1) / 2.0,
+        "chain": (40.0 + 8.0) * hh + (34.0 + 4.0) * nh * nH,
+    }
+
+
+def cpu_baseline(system, calls, threads):
+    import oracle
+    work = tempfile.mkdtemp(prefix="agb_cpu_")
+    if os.path.exists(oracle.REF_DRIVER):
+        _, timing = oracle.run_reference(system, work, repeat=calls, warmup=1, threads=threads, timeout=900)
+        return dict(value=system.natoms / timing["median_s"], unit="atom-evals/s", cores=timing["threads"],
+                    kind="reference",
+                    sample=f"{calls} timed agbnp3_ener calls (+1 warm-up) of the same {system.natoms}-atom solute, "
+                           f"reference library gcc -O3 -fopenmp -DUSE_SSE, median {timing['median_s']:.3f} s/call")
+    # no reference build on this host: the FP64 oracle restatement instead
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    o = oracle.Oracle(system)
+    t0 = time.perf_counter()
+    o.ener()
+    dt = time.perf_counter() - t0
+    return dict(value=system.natoms / dt, unit="atom-evals/s", cores=threads, kind="port",
+                sample=f"1 oracle evaluation of the same {system.natoms}-atom solute, {dt:.3f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--natoms", type=int, default=25000)
+    ap.add_argument("--seed", type=int, default=20260025)
+    ap.add_argument("--cpu-calls", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build evaluates one instance on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from agbnp3_amd import api, hip, synth
+
+    log(f"bench: generating {args.natoms}-atom solute (seed {args.seed})")
+    s = synth.generate(args.seed, args.natoms)
+    n = s.natoms
+    inst = api.Instance(s)
+    nh = inst.nheavy()
+    host_out = inst.ener()            # first call: sizes capacities, checks device status
+    log("bench: first host-API call done")
+
+    dev = {k: hip.DeviceArray.from_numpy(getattr(s, k)) for k in ("x", "y", "z")}
+    outs = {k: hip.DeviceArray(8 * n) for k in ("sp", "br", "sa")}
+    outs.update({k: hip.DeviceArray(24 * n) for k in ("dgbdr", "dvwdr", "decav", "dehb")})
+    outs["energies"] = hip.DeviceArray(8 * 7)
+    argv = [dev["x"].addr, dev["y"].addr, dev["z"].addr] + [outs[k].addr for k in (
+        "sp", "br", "sa", "dgbdr", "dvwdr", "decav", "dehb", "energies")]
+
+    def step():
+        if inst.ener_device(*argv) != api.AGBNP_OK:
+            raise RuntimeError("agbnp3_ener_device failed")
+
+    for _ in range(args.warmup):
+        step()
+    if inst.sync() != api.AGBNP_OK:
+        raise RuntimeError("device status error during warm-up")
+    inst.kernel_times()                # clears any earlier records
+    inst.set_options(api.OPT_TIMING)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    rc = inst.sync()
+    hip.synchronize()
+    t1 = time.perf_counter()
+    if rc != api.AGBNP_OK:
+        raise RuntimeError("device status error in the timed steps")
+    ktimes = inst.kernel_times()
+    inst.set_options(0)
+    elapsed = t1 - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n * args.steps / elapsed
+    e_dev = outs["energies"].to_numpy(np.float64, (7,))
+    same = all(e_dev[i] == host_out[k] for i, k in enumerate(api.ENERGY_NAMES))
+    log(f"bench: {ms_per_step:.3f} ms/step, {value:.4g} atom-evals/s; device outputs equal host-API call: {same}")
+    log("bench: kernel ms " + ", ".join(f"{k}={v:.3f}" for k, v in ktimes.items()))
+
+    # PCIe-inclusive host API rate (reported beside, never as value)
+    th0 = time.perf_counter()
+    for _ in range(3):
+        inst.ener()
+    host_rate = 3 * n / (time.perf_counter() - th0)
+
+    flops = algorithmic_flops(n, nh)
+    timed = {k: ktimes[k] for k in flops if k in ktimes}
+    dom = max(timed, key=timed.get)
+    achieved = flops[dom] / (timed[dom] * 1e-3)
+    roofline = dict(kernel=dom, bound="valu-fp32", achieved=achieved / 1e12, peak=PEAK_FP32 / 1e12,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP32, traffic=None,
+                    avg_ms=timed[dom], flops_per_launch=flops[dom])
+    total_flops = sum(flops.values())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"bench: reference CPU baseline, {threads} threads, {args.cpu_calls} calls")
+        cpu = cpu_baseline(s, args.cpu_calls, threads)
+
+    line = dict(
+        metric=METRIC, value=value, unit="atom-evals/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32 pairs, f64 reductions", data="synthetic",
+        config=dict(workload="config 1: plain agbnp3_ener E+grad with neighbour-list rebuild, one synthetic "
+                             f"globular solute of {n} atoms (seed {args.seed}), inputs resident in HBM",
+                    natoms=n, nheavy=nh, instances=1),
+        roofline=roofline, cpu_baseline=cpu,
+        pair_tflops=total_flops / (ms_per_step * 1e-3) / 1e12,
+        host_api_atom_evals_per_s=host_rate,
+        kernel_ms=ktimes,
+    )
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,66 @@
+/* Additive extensions of libagbnp3 (no counterpart in the reference API;
+ * the reference entry points in agbnp3.h are unchanged).
+ *
+ * All functions return AGBNP_OK / AGBNP_ERR like the reference API and print
+ * the reason on stderr.  Device pointers are plain hipMalloc'd addresses;
+ * `stream` arguments are hipStream_t values passed as void*.
+ */
+#ifndef AGBNP3_AMD_H
+#define AGBNP3_AMD_H
+#include "agbnp3.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- options (per instance; default 0 = reference behaviour) ---------- */
+/* E_HB free volume subtracts spe of the current overlap partner instead of
+   the reference's stale one (agbnp3_cpu_simd_kernels.c:2854, 2859) */
+#define AGBNP3_OPT_HB_FIXED      1
+/* spline derivative with the textbook sign of the y''(k+1) term instead of
+   the reference's (agbnp3_utils.c:292-293, agbnp3_cpu_simd_kernels.c:964-965) */
+#define AGBNP3_OPT_SPLINE_FIXED  2
+/* record per-kernel HIP event times of every call (agbnp3_kernel_times) */
+#define AGBNP3_OPT_TIMING        4
+int agbnp3_set_options(int tag, int options);
+
+/* ---- device-resident evaluation ---------------------------------------- */
+/* Same computation as agbnp3_ener with every array in device memory (caller
+   atom order): x, y, z [natoms]; sp, br, surf_area [natoms]; the four
+   gradients [3*natoms], interleaved like float_i (*)[3]; energies[7] =
+   egb, evdw, ecorr_vdw, ecav, ecorr_cav, ehb, mol_volume.  Enqueued on the
+   instance stream and returns without waiting; call agbnp3_sync (or wait on
+   agbnp3_stream) before reading outputs.  Host-side errors (bad tag, HB
+   topology) are reported immediately; device-side capacity errors by the
+   next agbnp3_sync. */
+int agbnp3_ener_device(int tag, const double *x, const double *y, const double *z,
+                       double *sp, double *br, double *surf_area,
+                       double *dgbdr, double *dvwdr, double *decav, double *dehb,
+                       double *energies);
+int agbnp3_sync(int tag);
+void *agbnp3_stream(int tag);
+
+/* ---- introspection of the last evaluation (tests, profiling) ----------- */
+int agbnp3_natoms(int tag, int *natoms, int *nheavy);
+/* internal (heavy-first) order: int2ext[natoms] */
+int agbnp3_get_int2ext(int tag, int *int2ext);
+/* heavy-atom neighbour list: nne[nheavy], rows padded to `cap` */
+int agbnp3_get_near_list(int tag, int *nne, int *nbr, int cap);
+/* Q4 spline tables: returns ntypes; y, y2 [ntypes*ntypes*512], radii[ntypes]
+   (any may be NULL) */
+int agbnp3_get_tables(int tag, float *y, float *y2, float *radii);
+/* per-atom internal array, internal order, natoms doubles: "volumep_raw",
+   "sa_raw", "sa_f", "sp", "spe", "psvol", "gammap", "beta", "br", "brw",
+   "swf", "A", "q2ab", "abrw", "deru", "derv", "derus", "dervs", "derh" */
+int agbnp3_get_internal(int tag, const char *name, double *out);
+/* counts[0] kept overlaps, [1] evaluated overlaps, [2] water sites,
+   [3] neighbour-list entries, [4] max row length */
+int agbnp3_get_counts(int tag, long long *counts);
+/* per-kernel times (ms), averaged over the calls made since the previous
+   read while AGBNP3_OPT_TIMING was set (HIP events on the instance stream);
+   returns the number of kernels written; names joined by ';' */
+int agbnp3_kernel_times(int tag, float *ms, int nmax, char *names, int names_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

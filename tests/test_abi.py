@@ -1,0 +1,53 @@
+"""The C-ABI boundary: headers, exported symbols, NeighList layout (CPU only,
+no compute calls)."""
+import ctypes
+import os
+import re
+
+
+from agbnp3_amd import _paths, api
+
+
+def declared_functions(header):
+    text = open(os.path.join(_paths.INCLUDE_DIR, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"^\s*(?:int|void|void \*)\s*\*?\s*(\w+)\s*\(", text, flags=re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    api.load()
+    lib = ctypes.CDLL(_paths.LIBAGBNP3)
+    nb = ctypes.CDLL(_paths.LIBNBLIST)
+    decl = declared_functions("agbnp3.h") | declared_functions("agbnp3_amd.h")
+    assert {"agbnp3_initialize", "agbnp3_new", "agbnp3_ener", "agbnp3_delete", "agbnp3_terminate",
+            "agbnp_initialize", "agbnp_terminate", "agbnp_delete"} <= decl
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(api.EXPORTED) == decl
+    for name in declared_functions("nblist.h"):
+        assert hasattr(nb, name), name
+    assert set(api.NBLIST_EXPORTED) == declared_functions("nblist.h")
+
+
+def test_library_is_built_for_gfx950():
+    # the offload bundle of our own build names its target triple
+    blob = open(_paths.LIBAGBNP3, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_neighlist_struct_matches_c_layout():
+    # nblist.h:39-70 on LP64: 2 ints, 3 pointers, int, 2 pointers, int, 2 pointers, int, 2 pointers
+    assert ctypes.sizeof(api.NeighList) == 8 + 24 + 8 + 16 + 8 + 16 + 8 + 16
+    assert api.NeighList.neighl.offset == 24
+    assert api.NeighList.data_index.offset == ctypes.sizeof(api.NeighList) - 8
+
+
+def test_nblist_roundtrip_on_cpu():
+    import numpy as np
+    nne = np.array([2, 1, 1], np.int32)
+    flat = np.array([1, 2, 0, 0], np.int32)
+    nl = api.make_conntbl(nne, flat)
+    assert nl.natoms == 3 and nl.neighl_size == 4
+    assert [nl.neighl[0][k] for k in range(2)] == [1, 2]
+    assert nl.neighl[2][0] == 0
+    api._nb.nblist_delete_neighbor_list(ctypes.byref(nl))

@@ -1,0 +1,209 @@
+"""HIP path (through the C-ABI) against the FP64 oracle and the reference's
+golden vectors.
+
+Tolerances (north star): each energy term within 1e-5 relative of the FP64
+oracle, gradients within 1e-4 kcal/mol/A RMS; the neighbour list and the
+spline tables bit-exact.  Against the reference (itself FP32 with SSE
+approximations) the bar is its own measured distance from the oracle.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from agbnp3_amd import api, synth
+from conftest import GOLDEN_CASES, load_golden
+
+pytestmark = pytest.mark.gpu
+
+TERMS = ("egb", "evdw", "ecorr_vdw", "ecav", "ecorr_cav", "ehb", "mol_volume")
+GRADS = ("dgbdr", "dvwdr", "decav", "dehb")
+E_REL = 1e-5
+G_RMS = 1e-4
+
+
+def rms(a):
+    return float(np.sqrt(np.mean(np.square(a))))
+
+
+def check_against_oracle(out, ref, e_rel=E_REL, g_rms=G_RMS, label=""):
+    for k in TERMS:
+        if ref[k] == 0.0:
+            assert abs(out[k]) < 1e-9, (label, k, out[k])
+        else:
+            assert abs(out[k] - ref[k]) <= e_rel * abs(ref[k]), (label, k, out[k], ref[k])
+    for k in GRADS:
+        assert rms(out[k] - ref[k]) <= g_rms, (label, k, rms(out[k] - ref[k]))
+    for k in ("sp", "br", "surf_area"):
+        assert rms(out[k] - ref[k]) <= 1e-4 * max(1.0, rms(ref[k])), (label, k)
+
+
+@pytest.fixture(scope="module")
+def golden_runs():
+    runs = {}
+    for case in GOLDEN_CASES:
+        s, ref = load_golden(case)
+        inst = api.Instance(s)
+        out = inst.ener()
+        orc = oracle.Oracle(s)
+        runs[case] = (s, ref, inst, out, orc, orc.ener())
+    yield runs
+    for v in runs.values():
+        v[2].delete()
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_energies_and_gradients_match_fp64_oracle(golden_runs, case):
+    s, ref, inst, out, orc, oref = golden_runs[case]
+    check_against_oracle(out, oref, label=case)
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_matches_reference_golden_vectors(golden_runs, case):
+    s, ref, inst, out, orc, oref = golden_runs[case]
+    for k in TERMS:
+        if ref[k] == 0.0:
+            assert abs(out[k]) < 1e-9
+        else:
+            assert abs(out[k] - ref[k]) <= 5e-6 * abs(ref[k]), (k, out[k], ref[k])
+    for k in GRADS:
+        assert rms(out[k] - ref[k]) <= 2e-3, (k, rms(out[k] - ref[k]))
+
+
+@pytest.mark.parametrize("case", ("c0_n2000", "n600_s11"))
+def test_neighbour_list_is_bit_exact(golden_runs, case):
+    s, ref, inst, out, orc, oref = golden_runs[case]
+    assert np.array_equal(inst.int2ext(), orc.int2ext())
+    nne_o, rows_o = orc.near_list()
+    nne_g, rows_g = inst.near_list(cap=256)
+    assert np.array_equal(nne_g, nne_o)
+    for i in range(len(nne_o)):
+        assert np.array_equal(rows_g[i, :nne_o[i]], rows_o[i, :nne_o[i]]), i
+
+
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+def test_spline_tables_are_bit_exact(golden_runs, case):
+    s, ref, inst, out, orc, oref = golden_runs[case]
+    yg, y2g, rg = inst.tables()
+    yo, y2o, ro = orc.tables()
+    assert np.array_equal(rg, ro)
+    assert np.array_equal(yg, yo)
+    assert np.array_equal(y2g, y2o)
+
+
+def test_internal_arrays_match_oracle(golden_runs):
+    s, ref, inst, out, orc, oref = golden_runs["c0_n2000"]
+    for name, tol in (("volumep_raw", 1e-5), ("sa_raw", 1e-4), ("sp", 1e-6), ("beta", 1e-6), ("A", 1e-5),
+                      ("q2ab", 1e-4), ("deru", 1e-5), ("derv", 1e-7), ("derh", 1e-5)):
+        g, o = inst.internal(name), orc.internal(name)
+        assert np.max(np.abs(g - o)) <= tol * max(1.0, np.max(np.abs(o))), name
+
+
+def test_overlap_tree_size_matches_oracle(golden_runs):
+    s, ref, inst, out, orc, oref = golden_runs["c0_n2000"]
+    cg, co = inst.counts(), orc.counts()
+    assert cg["near_pairs"] == co["near_pairs"]
+    assert cg["water_sites"] == co["water_sites"]
+    # pruning at V' > FLT_MIN may differ only for overlaps at the 0.01 edge
+    assert abs(cg["kept_overlaps"] - co["kept_overlaps"]) <= 2
+    assert abs(cg["evaluated_overlaps"] - co["evaluated_overlaps"]) <= 20
+
+
+@pytest.mark.parametrize("opts", (api.OPT_HB_FIXED, api.OPT_SPLINE_FIXED, api.OPT_HB_FIXED | api.OPT_SPLINE_FIXED))
+def test_corrected_modes_match_oracle(opts):
+    s, _ = load_golden("n600_s11")
+    inst = api.Instance(s, options=opts)
+    try:
+        out = inst.ener()
+    finally:
+        inst.delete()
+    oref = oracle.Oracle(s).ener(hb_fixed=bool(opts & api.OPT_HB_FIXED),
+                                 spline_fixed=bool(opts & api.OPT_SPLINE_FIXED))
+    check_against_oracle(out, oref, label=str(opts))
+
+
+def test_repeated_calls_are_bitwise_identical(golden_runs):
+    s, ref, inst, out, orc, oref = golden_runs["c0_n2000"]
+    again = inst.ener()
+    for k in TERMS:
+        assert again[k] == out[k], k
+    for k in GRADS + ("sp", "br", "surf_area"):
+        assert np.array_equal(again[k], out[k]), k
+
+
+def test_instances_are_independent():
+    s1 = synth.generate(101, 900)
+    s2 = synth.generate(102, 1300)
+    a = api.Instance(s1)
+    b = api.Instance(s2)
+    try:
+        ea1 = a.ener()
+        eb1 = b.ener()
+        ea2 = a.ener()
+        assert ea1["egb"] == ea2["egb"] and np.array_equal(ea1["dgbdr"], ea2["dgbdr"])
+        check_against_oracle(eb1, oracle.Oracle(s2).ener(), label="second instance")
+    finally:
+        a.delete()
+        b.delete()
+
+
+def test_delete_frees_the_slot_for_reuse():
+    s = synth.generate(7, 300)
+    a = api.Instance(s)
+    tag = a.tag
+    a.delete()
+    assert api.agbnp3_delete(tag) == api.AGBNP_ERR      # deleted tag is invalid
+    b = api.Instance(s)
+    try:
+        assert b.tag == tag                             # lowest free slot
+        check_against_oracle(b.ener(), oracle.Oracle(s).ener())
+    finally:
+        b.delete()
+
+
+def test_trigonal_oop_topology_error_fails_the_call():
+    s = synth.generate(7, 300)
+    oop = np.nonzero(s.hbtype == 13)[0]
+    assert oop.size
+    hb = s.hbtype.copy()
+    # a TRIGONAL_OOP acceptor on an atom with the wrong bond count
+    one_bond = np.nonzero((s.conn_nne == 1) & (s.hbtype == 0))[0][0]
+    hb[one_bond] = 13
+    bad = dataclasses.replace(s, hbtype=hb)
+    inst = api.Instance(bad)
+    try:
+        rc, _ = api.agbnp3_ener(inst.tag, 0, bad.x, bad.y, bad.z)
+        assert rc == api.AGBNP_ERR
+    finally:
+        inst.delete()
+
+
+def test_moderate_system_matches_oracle():
+    s = synth.generate(20265000, 6000)
+    inst = api.Instance(s)
+    try:
+        out = inst.ener()
+    finally:
+        inst.delete()
+    check_against_oracle(out, oracle.Oracle(s).ener(), label="n6000")
+
+
+def test_config1_size_invariants():
+    # 25,000 atoms: size-independent properties (the FP64 oracle is too slow here)
+    s = synth.generate(20260025, 25000)
+    inst = api.Instance(s)
+    try:
+        out = inst.ener()
+        shift = np.array([1.25, -0.5, 0.75])
+        moved = inst.ener(s.x + shift[0], s.y + shift[1], s.z + shift[2])
+    finally:
+        inst.delete()
+    for k in TERMS:
+        if out[k] != 0.0:
+            assert abs(moved[k] - out[k]) <= 1e-5 * abs(out[k]), k
+    for k in ("dgbdr", "dvwdr", "decav"):
+        net = out[k].sum(0)
+        assert np.abs(net).max() <= 1e-3 * max(1.0, np.abs(out[k]).max()) * np.sqrt(s.natoms) / 10, k
+    assert np.all(np.isfinite(out["dgbdr"]))
+    assert out["egb"] < 0 and out["ecav"] > 0 and out["mol_volume"] > 0
