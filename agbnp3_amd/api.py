@@ -27,7 +27,7 @@ EXPORTED = (
     "agbnp_initialize", "agbnp_terminate", "agbnp_delete",
     "agbnp3_set_options", "agbnp3_ener_device", "agbnp3_sync", "agbnp3_stream", "agbnp3_natoms",
     "agbnp3_get_int2ext", "agbnp3_get_near_list", "agbnp3_get_tables", "agbnp3_get_internal",
-    "agbnp3_get_counts", "agbnp3_kernel_times",
+    "agbnp3_get_counts", "agbnp3_kernel_times", "agbnp3_q4_tables",
 )
 NBLIST_EXPORTED = ("nblist_reset_neighbor_list", "nblist_delete_neighbor_list", "nblist_reallocate_neighbor_list")
 
@@ -86,6 +86,7 @@ def load():
         L.agbnp3_get_internal.argtypes = [ci, ctypes.c_char_p, vp]
         L.agbnp3_get_counts.argtypes = [ci, vp]
         L.agbnp3_kernel_times.argtypes = [ci, vp, ci, ctypes.c_char_p, ci]
+        L.agbnp3_q4_tables.argtypes = [ci, vp, vp, vp, vp]
         _nb.nblist_reallocate_neighbor_list.argtypes = [ctypes.POINTER(NeighList), ci, ci]
         _nb.nblist_reset_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
         _nb.nblist_delete_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
@@ -164,6 +165,19 @@ def agbnp3_delete(tag):
 agbnp_initialize = agbnp3_initialize
 agbnp_terminate = agbnp3_terminate
 agbnp_delete = agbnp3_delete
+
+
+def q4_tables(radii):
+    """Host-side Q4 tables (no GPU): (y, y2, dx, dxinv) for radius types `radii`."""
+    L = load()
+    r = np.ascontiguousarray(radii, np.float32)
+    nt = r.size
+    y = np.zeros(nt * nt * 512, np.float32)
+    y2 = np.zeros_like(y)
+    dx = np.zeros(2, np.float32)
+    if L.agbnp3_q4_tables(nt, _p(r), _p(y), _p(y2), _p(dx)) != AGBNP_OK:
+        raise RuntimeError("agbnp3_q4_tables failed")
+    return y.reshape(nt * nt, 512), y2.reshape(nt * nt, 512), float(dx[0]), float(dx[1])
 
 
 # ------------------------------------------------------------ convenience

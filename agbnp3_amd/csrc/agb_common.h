@@ -97,6 +97,7 @@ struct Dev {
   float tdx, tdxinv;
   const int *row_perm;   // type-grouped row order, -1 padded
   const int *block_type; // row type of each 256-row block
+  const int *block_hrows;  // 1 if every row of the block is a hydrogen
   int nrow_blocks;
   const WsDef *wsdef;
   const int *ws_stale0;  // per site: atom whose spe the first subtraction reads

@@ -45,7 +45,8 @@ void errprint(const char *fmt, ...) {
 // Gaussian exponents taken from Ri), agbnp3_create_ctablef4
 // (agbnp3_utils.c:380-422: 511 nodes on [0, 20) at an accumulated spacing,
 // y[511] = 0) and agbnp3_cspline_setup (agbnp3_utils.c:193-227, whose end
-// condition divides by 2 dx).
+// condition divides by 2 dx).  The reference is C, where log/exp/pow of a
+// float promote to double; the explicit casts keep that under C++ overloads.
 float q4_integral(float rij, float Ri, float Rj, float *dr) {
   const float twopi = 2.0 * kPi;
   const float twothirds = 2.0 / 3.0;
@@ -53,7 +54,7 @@ float q4_integral(float rij, float Ri, float Rj, float *dr) {
   float q;
   if (rij > (Ri + Rj)) {                         // no overlap
     float u1 = rij + Rj, u2 = rij - Rj, u3 = u1 * u2;
-    float u4 = 0.5 * log(u1 / u2);
+    float u4 = 0.5 * log((double)(u1 / u2));
     q = twopi * (Rj / u3 - u4 / rij);
     *dr = twopi * ((Rj / (rij * u3)) * (1. - 2. * rij2 / u3) + u4 / rij2);
     return q;
@@ -64,7 +65,7 @@ float q4_integral(float rij, float Ri, float Rj, float *dr) {
     float u2 = rij - Rj, u3 = u1 * u2;
     float u4 = 1. / u1, u4sq = u4 * u4;
     float u5 = 1. / Ri, u5sq = u5 * u5;
-    float u6 = 0.5 * log(u1 / Ri);
+    float u6 = 0.5 * log((double)(u1 / Ri));
     q = twopi * (-(u4 - u5) + (0.25 * u3 * (u4sq - u5sq) - u6) / rij);
     *dr = twopi * (0.5 * (1. - 0.5 * u3 / rij2) * (u4sq - u5sq) + u6 / rij2);
     return q;
@@ -81,7 +82,7 @@ float q4_integral(float rij, float Ri, float Rj, float *dr) {
     q = twopi * (2. / Ri + Rj / u3 - u6);
     *dr = -(2. * twopi * a / (Rj * Rj)) * (1. / ad + twothirds);
   } else {
-    float u6 = 0.5 * log(u1 / u2);
+    float u6 = 0.5 * log((double)(u1 / u2));
     q = twopi * (2. / Ri + Rj / u3 - u6 / (rij));
     *dr = twopi * (-(Rj / u3) * (2. * rij / u3 - 1. / (rij)) + u6 / (rij2));
   }
@@ -94,7 +95,7 @@ float q4_inflated(float rij, float Ri, float Rj, float *dr) {
   const float d2 = rij * rij;
   const float deltai = 1. / (ai + ai);
   const float p = pfc * pfc;
-  const float kappa = exp(-ai * ai * d2 * deltai);
+  const float kappa = exp((double)(-ai * ai * d2 * deltai));
   const float gvol = p * kappa * pow(kPi * deltai, 1.5);
   const float volj = 4. * kPi * Rj * Rj * Rj / 3.;
   const float newRj = pow((volj + 2. * gvol) / volj, 1. / 3.) * Rj;
@@ -150,7 +151,7 @@ struct Instance {
   std::vector<float> radii, ty, ty2;
   float tdx = 0, tdxinv = 0;
   float eps_in = 1, eps_out = 1;
-  std::vector<int> row_perm, block_type;
+  std::vector<int> row_perm, block_type, block_hrows;
   std::vector<WsDef> wsdef;
   std::vector<int> ws_stale0;
   int nws = 0;
@@ -344,17 +345,26 @@ int setup_device(Instance *I) {
   // rows grouped by radius type, 256-row blocks of a single type
   I->row_perm.clear();
   I->block_type.clear();
+  I->block_hrows.clear();
   for (int t = 0; t < I->ntypes; t++) {
     std::vector<int> rows;
     for (int i = 0; i < n; i++)
       if (I->rtype[i] == t) rows.push_back(i);
     for (size_t b = 0; b < rows.size(); b += 256) {
-      for (size_t k = 0; k < 256; k++) I->row_perm.push_back(b + k < rows.size() ? rows[b + k] : -1);
+      int allh = 1;
+      for (size_t k = 0; k < 256; k++) {
+        const int r = b + k < rows.size() ? rows[b + k] : -1;
+        I->row_perm.push_back(r);
+        if (r >= 0 && r < nh) allh = 0;
+      }
       I->block_type.push_back(t);
+      I->block_hrows.push_back(allh);
     }
   }
   d.nrow_blocks = (int)I->block_type.size();
-  if (upload(I, &d.row_perm, I->row_perm) || upload(I, &d.block_type, I->block_type)) return AGBNP_ERR;
+  if (upload(I, &d.row_perm, I->row_perm) || upload(I, &d.block_type, I->block_type) ||
+      upload(I, &d.block_hrows, I->block_hrows))
+    return AGBNP_ERR;
   d.nwsdef = (int)I->wsdef.size();
   d.nws = I->nws;
   d.ws_cap = 128;
@@ -736,6 +746,23 @@ int agbnp3_ener(int tag, int init, float_i *x, float_i *y, float_i *z, float_i *
   *ecorr_cav = e[4];
   *ehb = e[5];
   *mol_volume = e[6];
+  return AGBNP_OK;
+}
+
+int agbnp3_q4_tables(int ntypes, const float *radii, float *y, float *y2, float *dx) {
+  if (ntypes <= 0 || !radii) return AGBNP_ERR;
+  float tdx = 0.f, tdxinv = 0.f;
+  for (int a = 0; a < ntypes; a++)
+    for (int b = 0; b < ntypes; b++) {
+      const float c = kRadInc;
+      const float ratio = (radii[a] - c) / radii[b];
+      const size_t t = (size_t)(a * ntypes + b) * kTblN;
+      std::vector<float> ty(kTblN), ty2(kTblN);
+      build_table(ratio, ty.data(), ty2.data(), tdx, tdxinv);
+      if (y) memcpy(y + t, ty.data(), sizeof(float) * kTblN);
+      if (y2) memcpy(y2 + t, ty2.data(), sizeof(float) * kTblN);
+    }
+  if (dx) { dx[0] = tdx; dx[1] = tdxinv; }
   return AGBNP_OK;
 }
 

@@ -275,6 +275,7 @@ constexpr int kWsWaves = 4;
 constexpr int kWsCap = 128;
 
 __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
+#pragma clang fp contract(off)
   __shared__ int cj[kWsWaves][kWsCap];
   __shared__ int cs[kWsWaves][kWsCap];        // sorted candidate atoms
   __shared__ float cvp[kWsWaves][kWsCap];

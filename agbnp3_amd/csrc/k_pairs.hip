@@ -172,9 +172,9 @@ __global__ __launch_bounds__(kRows) void k_chain(Dev d) {
     spk = kheavy ? d.sp_f[kr] : 0.f;
   }
   const float iRk = 1.0f / pk.w;
-  // hydrogen rows only pair with heavy columns; a block's rows share a type,
-  // but heavy and hydrogen rows of equal radius may share a block
-  const int ncol = d.n;
+  // hydrogen rows only pair with heavy columns (heavy atoms come first);
+  // heavy and hydrogen rows of equal radius may share a block
+  const int ncol = d.block_hrows[b] ? d.nh : d.n;
   const int per = (ncol + nchunk - 1) / nchunk;
   const int c0 = chunk * per, c1 = min(ncol, c0 + per);
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;

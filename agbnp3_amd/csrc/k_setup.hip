@@ -171,6 +171,7 @@ constexpr int kNbWaves = 4;
 
 template <int CAP>
 __global__ __launch_bounds__(64 * kNbWaves) void k_nblist(Dev d) {
+#pragma clang fp contract(off)
   __shared__ float sd2[kNbWaves][CAP];
   __shared__ int sj[kNbWaves][CAP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

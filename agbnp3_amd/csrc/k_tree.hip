@@ -13,9 +13,10 @@
 // over sibling groups, each group evaluated lane-parallel and compacted in
 // order with ballots.  Every overlap of the root involves only slot atoms,
 // so contributions are reduced per slot in LDS and scattered to atoms once
-// per root with order-independent fixed-point atomics.  Contributions to the
-// atoms on the current path are shared by all lanes: they are summed in lane
-// registers and reduced across the wave when the path advances.
+// per root with order-independent fixed-point atomics.  Within the wave,
+// member contributions are added to FP64 LDS slots with ds_add_f64; only the
+// root, a member of every overlap, is summed in lane registers and reduced
+// once per root.
 //
 // Pass B re-walks the identical tree (same FP32 arithmetic, same pruning)
 // instead of storing it, and applies in one place the derivatives of
@@ -39,9 +40,8 @@ struct TreeLds {
   // sibling groups of depths 0..2 (overlap Gaussian + last slot)
   float ga[3][CAP], gp[3][CAP], gx[3][CAP], gy[3][CAP], gz[3][CAP];
   int gs[3][CAP];
-  // per-slot accumulators: pass A (volume, surface) FP64, pass B 12 FP32
-  double accA[PB ? 1 : 2][CAP + 1];
-  float accB[PB ? TG_N : 1][CAP + 1];
+  // per-slot FP64 accumulators: pass A (volume, surface), pass B 12 gradient terms
+  double acc[PB ? TG_N : 2][CAP + 1];
 };
 
 // coefficients of an overlap of order n
@@ -61,7 +61,7 @@ __device__ __forceinline__ OvlOut overlap(const Gauss &par, float ab, float xb, 
   const float d2 = dx * dx + dy * dy + dz * dz;
   const float a12 = par.a + ab;
   const float dinv = 1.0f / a12;
-  const float p12 = par.p * kPFC * __expf(-par.a * ab * d2 * dinv);
+  const float p12 = par.p * kPFC * expf(-par.a * ab * d2 * dinv);
   const float u = kPiF * dinv;
   o.V = p12 * u * sqrtf(u);
   float s, sp, spp;
@@ -92,8 +92,7 @@ __device__ __forceinline__ float sa_term(const OvlOut &o, float sign, float a, f
 template <int L, int CAP, bool PB, bool STORE>
 __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &par, const int *path,
                                              int ncand, const int *cand_slots_lds, int cand_base,
-                                             double (&pa)[4][2], float (&pb)[4][TG_N], float df,
-                                             unsigned &nkept, unsigned &neval) {
+                                             double (&root)[TG_N], float df, unsigned &nkept, unsigned &neval) {
   constexpr int N = L + 2;
   const int lane = threadIdx.x & 63;
   const float sign = ovl_sign(N);
@@ -132,13 +131,15 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
       const float ab = S.a[sb], xb = S.x[sb], yb = S.y[sb], zb = S.z[sb], Rb = S.R[sb];
       if (!PB) {
         const double wv = (double)cn * (double)o.Vp;
+        root[0] += wv;
+        root[1] += (double)sa_term(o, sign, ma[0], mx[0], my[0], mz[0], mR[0]);
 #pragma unroll
-        for (int k = 0; k < N - 1; k++) {
-          pa[k][0] += wv;
-          pa[k][1] += (double)sa_term(o, sign, ma[k], mx[k], my[k], mz[k], mR[k]);
+        for (int k = 1; k < N - 1; k++) {
+          atomicAdd(&S.acc[0][path[k]], wv);
+          atomicAdd(&S.acc[1][path[k]], (double)sa_term(o, sign, ma[k], mx[k], my[k], mz[k], mR[k]));
         }
-        S.accA[0][sb] += wv;
-        S.accA[1][sb] += (double)sa_term(o, sign, ab, xb, yb, zb, Rb);
+        atomicAdd(&S.acc[0][sb], wv);
+        atomicAdd(&S.acc[1][sb], (double)sa_term(o, sign, ab, xb, yb, zb, Rb));
       } else {
         // members: path[0..N-2] then the candidate (index N-1)
         float A_[N], X_[N], Y_[N], Z_[N], R_[N], WC[N], WV[N], WG[N];
@@ -190,12 +191,13 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
             out[TG_CAV + q] = g * bc + ha * (S2c[q] - WC[k] * g * iR);
             out[TG_HB + q] = Hc * g;
           }
-          if (k < N - 1) {
+          if (k == 0) {
 #pragma unroll
-            for (int v = 0; v < TG_N; v++) pb[k][v] += out[v];
+            for (int v = 0; v < TG_N; v++) root[v] += (double)out[v];
           } else {
+            const int slot = k < N - 1 ? path[k] : sb;
 #pragma unroll
-            for (int v = 0; v < TG_N; v++) S.accB[v][sb] += out[v];
+            for (int v = 0; v < TG_N; v++) atomicAdd(&S.acc[v][slot], (double)out[v]);
           }
         }
       }
@@ -211,26 +213,6 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
   nkept += (unsigned)cnt;
   __syncthreads();
   return cnt;
-}
-
-// reduce the lane accumulators of path depth k into the LDS slot `slot`
-template <int CAP, bool PB>
-__device__ __forceinline__ void flush_path(TreeLds<CAP, PB> &S, int k, int slot, double (&pa)[4][2],
-                                           float (&pb)[4][TG_N]) {
-  const int lane = threadIdx.x & 63;
-  if (!PB) {
-    const double v0 = wave_sum(pa[k][0]), v1 = wave_sum(pa[k][1]);
-    if (lane == 0) { S.accA[0][slot] += v0; S.accA[1][slot] += v1; }
-    pa[k][0] = pa[k][1] = 0.0;
-  } else {
-#pragma unroll
-    for (int v = 0; v < TG_N; v++) {
-      const float r = wave_sum(pb[k][v]);
-      if (lane == 0) S.accB[v][slot] += r;
-      pb[k][v] = 0.f;
-    }
-  }
-  __syncthreads();
 }
 
 template <int CAP, bool PB>
@@ -249,66 +231,53 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
     if (PB) {
       S.u[s] = d.deru_f[at]; S.v[s] = d.derv_f[at]; S.h[s] = d.derh_f[at];
       S.wc[s] = d.gammap_f[at]; S.wv[s] = d.dervs_f[at]; S.wg[s] = d.derus_f[at];
-#pragma unroll
-      for (int v = 0; v < TG_N; v++) S.accB[v][s] = 0.f;
-    } else {
-      S.accA[0][s] = 0.0;
-      S.accA[1][s] = 0.0;
     }
+#pragma unroll
+    for (int v = 0; v < (PB ? TG_N : 2); v++) S.acc[v][s] = 0.0;
   }
   __syncthreads();
-  double pa[4][2];
-  float pb[4][TG_N];
+  double root[TG_N];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    pa[k][0] = pa[k][1] = 0.0;
-#pragma unroll
-    for (int v = 0; v < TG_N; v++) pb[k][v] = 0.f;
-  }
+  for (int v = 0; v < TG_N; v++) root[v] = 0.0;
   unsigned nkept = 0, neval = 0;
   const float df = d.eps_df;
   int path[4];
   path[0] = 0;
-  const Gauss root = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
+  const Gauss rg = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
   // order 2: (i, j) for every list entry
-  const int n0 = eval_children<0, CAP, PB, true>(S, root, path, m, nullptr, 1, pa, pb, df, nkept, neval);
-  for (int i0 = 0; i0 < n0; i0++) {
+  const int n0 = eval_children<0, CAP, PB, true>(S, rg, path, m, nullptr, 1, root, df, nkept, neval);
+  for (int i0 = 0; i0 + 1 < n0; i0++) {
     path[1] = S.gs[0][i0];
-    if (i0 + 1 < n0) {
-      const Gauss g0 = {S.ga[0][i0], S.gp[0][i0], S.gx[0][i0], S.gy[0][i0], S.gz[0][i0]};
-      const int n1 = eval_children<1, CAP, PB, true>(S, g0, path, n0 - i0 - 1, &S.gs[0][0], i0 + 1, pa, pb, df,
+    const Gauss g0 = {S.ga[0][i0], S.gp[0][i0], S.gx[0][i0], S.gy[0][i0], S.gz[0][i0]};
+    const int n1 = eval_children<1, CAP, PB, true>(S, g0, path, n0 - i0 - 1, &S.gs[0][0], i0 + 1, root, df, nkept,
+                                                   neval);
+    for (int i1 = 0; i1 + 1 < n1; i1++) {
+      path[2] = S.gs[1][i1];
+      const Gauss g1 = {S.ga[1][i1], S.gp[1][i1], S.gx[1][i1], S.gy[1][i1], S.gz[1][i1]};
+      const int n2 = eval_children<2, CAP, PB, true>(S, g1, path, n1 - i1 - 1, &S.gs[1][0], i1 + 1, root, df,
                                                      nkept, neval);
-      for (int i1 = 0; i1 < n1; i1++) {
-        path[2] = S.gs[1][i1];
-        if (i1 + 1 < n1) {
-          const Gauss g1 = {S.ga[1][i1], S.gp[1][i1], S.gx[1][i1], S.gy[1][i1], S.gz[1][i1]};
-          const int n2 = eval_children<2, CAP, PB, true>(S, g1, path, n1 - i1 - 1, &S.gs[1][0], i1 + 1, pa, pb,
-                                                         df, nkept, neval);
-          for (int i2 = 0; i2 < n2; i2++) {
-            path[3] = S.gs[2][i2];
-            if (i2 + 1 < n2) {
-              const Gauss g2 = {S.ga[2][i2], S.gp[2][i2], S.gx[2][i2], S.gy[2][i2], S.gz[2][i2]};
-              eval_children<3, CAP, PB, false>(S, g2, path, n2 - i2 - 1, &S.gs[2][0], i2 + 1, pa, pb, df, nkept,
-                                               neval);
-            }
-            flush_path(S, 3, path[3], pa, pb);
-          }
-        }
-        flush_path(S, 2, path[2], pa, pb);
+      for (int i2 = 0; i2 + 1 < n2; i2++) {
+        path[3] = S.gs[2][i2];
+        const Gauss g2 = {S.ga[2][i2], S.gp[2][i2], S.gx[2][i2], S.gy[2][i2], S.gz[2][i2]};
+        eval_children<3, CAP, PB, false>(S, g2, path, n2 - i2 - 1, &S.gs[2][0], i2 + 1, root, df, nkept, neval);
       }
     }
-    flush_path(S, 1, path[1], pa, pb);
   }
-  flush_path(S, 0, 0, pa, pb);
-  // scatter the per-slot sums
+  // the root's own sums, then every slot to its atom
+#pragma unroll
+  for (int v = 0; v < (PB ? TG_N : 2); v++) {
+    const double r = wave_sum(root[v]);
+    if (lane == 0) S.acc[v][0] += r;
+  }
+  __syncthreads();
   for (int s = lane; s <= m; s += 64) {
     const int at = S.idx[s];
     if (!PB) {
-      fix_add(&d.acc_vol[at], S.accA[0][s]);
-      fix_add(&d.acc_sa[at], S.accA[1][s]);
+      fix_add(&d.acc_vol[at], S.acc[0][s]);
+      fix_add(&d.acc_sa[at], S.acc[1][s]);
     } else {
 #pragma unroll
-      for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], (double)S.accB[v][s]);
+      for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], S.acc[v][s]);
     }
   }
   if (!PB && lane == 0) {

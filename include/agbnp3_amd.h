@@ -39,6 +39,12 @@ int agbnp3_ener_device(int tag, const double *x, const double *y, const double *
 int agbnp3_sync(int tag);
 void *agbnp3_stream(int tag);
 
+/* ---- host-only helpers (no GPU needed) ------------------------------------ */
+/* Q4 spline tables for radius types `radii` (R = r + 0.5): y, y2 sized
+   ntypes*ntypes*512, table (i, j) at offset (i*ntypes + j)*512; dx[0..1] =
+   node spacing and its inverse.  The same tables agbnp3_new builds. */
+int agbnp3_q4_tables(int ntypes, const float *radii, float *y, float *y2, float *dx);
+
 /* ---- introspection of the last evaluation (tests, profiling) ----------- */
 int agbnp3_natoms(int tag, int *natoms, int *nheavy);
 /* internal (heavy-first) order: int2ext[natoms] */

@@ -1,9 +1,11 @@
-"""The C-ABI boundary: headers, exported symbols, NeighList layout (CPU only,
-no compute calls)."""
+"""The C-ABI boundary: headers, exported symbols, NeighList layout, host-side
+table builder (CPU only, no GPU calls)."""
 import ctypes
 import os
 import re
 
+
+import numpy as np
 
 from agbnp3_amd import _paths, api
 
@@ -51,3 +53,15 @@ def test_nblist_roundtrip_on_cpu():
     assert [nl.neighl[0][k] for k in range(2)] == [1, 2]
     assert nl.neighl[2][0] == 0
     api._nb.nblist_delete_neighbor_list(ctypes.byref(nl))
+
+
+def test_host_q4_tables_are_bit_identical_to_the_oracle():
+    import oracle
+    from conftest import GOLDEN_CASES, load_golden
+    for case in GOLDEN_CASES:
+        s, _ = load_golden(case)
+        yo, y2o, ro = oracle.Oracle(s).tables()
+        yg, y2g, dx, dxi = api.q4_tables(ro)
+        assert np.array_equal(yg, yo), case
+        assert np.array_equal(y2g, y2o), case
+        assert dx == np.float32(20.0) / np.float32(511)
