@@ -119,8 +119,10 @@ struct Dev {
   int nchunk_desc;
   double *beta, *br, *brw, *swf;       // [n]
   float *br_f;
-  double *gb_part;                     // [nchunk_gb][5][n]: A, gx, gy, gz, e
-  int nchunk_gb;
+  long long *acc_gbA, *acc_gbg;        // GB pairs: A_i [n], gradient [n][3] (fixed point)
+  double *gb_epart;                    // per tile pair: sum of q_i q_j / f
+  const int2 *tile_ij;                 // tile pairs (I <= J) of 256-atom blocks
+  int ntiles;
   double *A, *q2ab, *abrw;             // [n]
   float *q2ab_f, *abrw_f;
   double *ch_part;                     // [nchunk_ch][8][n]: gbx,gby,gbz,vwx,vwy,vwz,U,V

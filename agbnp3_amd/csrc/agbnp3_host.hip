@@ -372,7 +372,14 @@ int setup_device(Instance *I) {
   // per-call buffers
   const int nblk = (n + 255) / 256;
   d.nchunk_desc = choose_chunks(d.nrow_blocks, nh);
-  d.nchunk_gb = choose_chunks(nblk, n);
+  {
+    std::vector<int2> tiles;
+    const int nb = (n + 255) / 256;
+    for (int a = 0; a < nb; a++)
+      for (int b = a; b < nb; b++) tiles.push_back(make_int2(a, b));
+    d.ntiles = (int)tiles.size();
+    if (upload(I, &d.tile_ij, tiles)) return AGBNP_ERR;
+  }
   d.nchunk_ch = choose_chunks(d.nrow_blocks, n);
   d.epart_stride = nblk;
   const size_t nn = n > 0 ? n : 1, nhh = nh > 0 ? nh : 1;
@@ -386,7 +393,8 @@ int setup_device(Instance *I) {
       dalloc(I, &d.sp_f, nn) || dalloc(I, &d.spe_f, nn) ||
       dalloc(I, &d.beta_part, (size_t)d.nchunk_desc * nn) || dalloc(I, &d.beta, nn) || dalloc(I, &d.br, nn) ||
       dalloc(I, &d.brw, nn) || dalloc(I, &d.swf, nn) || dalloc(I, &d.br_f, nn) ||
-      dalloc(I, &d.gb_part, (size_t)d.nchunk_gb * 5 * nn) || dalloc(I, &d.A, nn) || dalloc(I, &d.q2ab, nn) ||
+      dalloc(I, &d.acc_gbA, nn) || dalloc(I, &d.acc_gbg, 3 * nn) ||
+      dalloc(I, &d.gb_epart, (size_t)std::max(1, d.ntiles)) || dalloc(I, &d.A, nn) || dalloc(I, &d.q2ab, nn) ||
       dalloc(I, &d.abrw, nn) || dalloc(I, &d.q2ab_f, nn) || dalloc(I, &d.abrw_f, nn) ||
       dalloc(I, &d.ch_part, (size_t)d.nchunk_ch * 8 * nn) || dalloc(I, &d.deru, nn) || dalloc(I, &d.derv, nn) ||
       dalloc(I, &d.derus, nn) || dalloc(I, &d.dervs, nn) || dalloc(I, &d.derh, nn) ||

@@ -116,29 +116,18 @@ void launch_born(const Dev &d, hipStream_t s) {
 int born_blocks(const Dev &d) { return (d.n + kEpi - 1) / kEpi; }
 
 // -------------------------------------------------- chain-rule factors
-// q2ab = (q^2 + A B) swf', abrw = alpha brw swf' (kernels.c:433-450);
-// also the pair-energy partial df * sum_j q_i q_j / f (pairs counted twice)
+// q2ab = (q^2 + A B) swf', abrw = alpha brw swf' (kernels.c:433-450)
 __global__ __launch_bounds__(kEpi) void k_q2ab(Dev d) {
   const int i = blockIdx.x * kEpi + threadIdx.x;
-  double epair = 0.0;
   if (i < d.n) {
-    double A = 0.0, e = 0.0;
-    for (int c = 0; c < d.nchunk_gb; c++) {
-      const double *p = d.gb_part + (size_t)c * 5 * d.n;
-      A += p[i];
-      e += p[4 * (size_t)d.n + i];
-    }
+    const double A = fix_get(d.acc_gbA[i]);
     const double q = d.q[i];
     d.A[i] = A;
     d.q2ab[i] = (q * q + A * d.br[i]) * d.swf[i];
     d.abrw[i] = ((double)d.ialpha[i] + (double)d.salpha[i]) * d.brw[i] * d.swf[i];
     d.q2ab_f[i] = (float)d.q2ab[i];
     d.abrw_f[i] = (float)d.abrw[i];
-    epair = d.df * e;
   }
-  const int slots[1] = {EP_PAIR};
-  const double v[1] = {epair};
-  block_partials(d, slots, v, 1);
 }
 
 void launch_q2ab(const Dev &d, hipStream_t s) {
@@ -188,10 +177,7 @@ __global__ __launch_bounds__(kEpi) void k_final(Dev d, double *sp, double *br, d
   const int e = d.int2ext[i];
   const size_t n = d.n;
   double g_gb[3] = {0, 0, 0}, g_vw[3] = {0, 0, 0};
-  for (int c = 0; c < d.nchunk_gb; c++) {
-    const double *p = d.gb_part + (size_t)c * 5 * n;
-    for (int q = 0; q < 3; q++) g_gb[q] += 2.0 * d.df * p[(1 + q) * n + i];
-  }
+  for (int q = 0; q < 3; q++) g_gb[q] = 2.0 * d.df * fix_get(d.acc_gbg[3 * (size_t)i + q]);
   const double f4 = 1.0 / (4.0 * kPi);
   for (int c = 0; c < d.nchunk_ch; c++) {
     const double *p = d.ch_part + (size_t)c * 8 * n;
@@ -223,16 +209,20 @@ __global__ __launch_bounds__(kEpi) void k_final(Dev d, double *sp, double *br, d
 }
 
 // fixed-order reduction of the per-block energy partials and site energies
-__global__ __launch_bounds__(256) void k_energy(Dev d, double *energies, int nscale, int nborn, int nq2ab) {
+__global__ __launch_bounds__(256) void k_energy(Dev d, double *energies, int nscale, int nborn) {
   __shared__ double red[EP_COUNT][4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double acc[EP_COUNT];
   for (int k = 0; k < EP_COUNT; k++) acc[k] = 0.0;
-  const int nb[EP_COUNT] = {nscale, nscale, nscale, nborn, nborn, nborn, nq2ab, 0};
-  for (int k = 0; k < EP_HB; k++) {
+  const int nb[EP_COUNT] = {nscale, nscale, nscale, nborn, nborn, nborn, 0, 0};
+  for (int k = 0; k < EP_PAIR; k++) {
     // contiguous ranges per thread keep the order fixed
     const int per = (nb[k] + 255) / 256;
     for (int b = t * per; b < min(nb[k], (t + 1) * per); b++) acc[k] += d.epart[(size_t)k * d.epart_stride + b];
+  }
+  {
+    const int per = (d.ntiles + 255) / 256;
+    for (int b = t * per; b < min(d.ntiles, (t + 1) * per); b++) acc[EP_PAIR] += d.df * d.gb_epart[b];
   }
   {
     const int per = (d.nws + 255) / 256;
@@ -260,8 +250,7 @@ void launch_final(const Dev &d, double *sp, double *br, double *sa, double *dgbd
                   double *dehb, double *energies, hipStream_t s) {
   hipLaunchKernelGGL(k_final, dim3((d.n + kEpi - 1) / kEpi), dim3(kEpi), 0, s, d, sp, br, sa, dgbdr, dvwdr, decav,
                      dehb);
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, d, energies, scale_blocks(d), born_blocks(d),
-                     (d.n + kEpi - 1) / kEpi);
+  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, d, energies, scale_blocks(d), born_blocks(d));
 }
 
 }  // namespace agb

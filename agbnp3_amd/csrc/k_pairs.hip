@@ -74,68 +74,95 @@ void launch_descreen(const Dev &d, hipStream_t s) {
 }
 
 // ------------------------------------------------------------- GB pairs
-// per row i, all j != i (both orders, so each pair is counted twice):
-//   e    = q_i q_j / f,  f = sqrt(d2 + B_i B_j exp(-d2 / 4 B_i B_j))
-//   g    = q_i q_j (1 - e/4) / f^3 (x_j - x_i)      (x 2 df later)
-//   A_i  = q_i q_j (B_i B_j + d2/4) exp(.) / f^3
-// (reference agbnp3_gb_energy_inner_nolist_soa, kernels.c:66-151)
-__global__ __launch_bounds__(kRows) void k_gbpair(Dev d) {
-  __shared__ float4 cpos[kTile];   // x, y, z, q
-  __shared__ float cb[kTile];      // B
-  const int i = blockIdx.x * kRows + threadIdx.x;
-  const int chunk = blockIdx.y, nchunk = gridDim.y;
-  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-  float qi = 0.f, bi = 1.f;
+// Each unordered pair once (agbnp3_gb_energy_inner_nolist_soa, kernels.c:66-151):
+//   e   = q_i q_j / f,  f = sqrt(d2 + B_i B_j exp(-d2 / 4 B_i B_j))
+//   g_i += q_i q_j (1 - e/4) / f^3 (x_j - x_i),  g_j -= same      (x 2 df later)
+//   A_i, A_j += q_i q_j (B_i B_j + d2/4) exp(.) / f^3
+// A block takes a tile pair (I <= J) of 256-atom blocks.  Each wave holds 64
+// row atoms in its lanes and sweeps the 256 column atoms 64 at a time: the
+// column atom, its data and its partial sums move one lane per step with a
+// DPP wave rotation, so every lane meets every column once without LDS
+// traffic.  Diagonal tiles see each pair twice and keep only the row side.
+__device__ __forceinline__ float wave_rot(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int wave_rot(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, false); }
+
+__global__ __launch_bounds__(256) void k_gbpair(Dev d) {
+  __shared__ float colacc[4][4][256];   // [wave][A, gx, gy, gz][column]: fixed-order sum
+  __shared__ double ered[4];
+  const int2 tp = d.tile_ij[blockIdx.x];
+  const bool diag = tp.x == tp.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = tp.x * 256 + threadIdx.x;
+  float xi = 1e6f + (float)threadIdx.x, yi = 0.f, zi = 0.f, qi = 0.f, bi = 1.f;   // padding: far, no charge
   if (i < d.n) {
-    pi = d.pos[i];
+    const float4 p = d.pos[i];
+    xi = p.x; yi = p.y; zi = p.z;
     qi = d.q[i];
     bi = d.br_f[i];
   }
-  const int per = (d.n + nchunk - 1) / nchunk;
-  const int c0 = chunk * per, c1 = min(d.n, c0 + per);
-  double A = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, e = 0.0;
-  for (int base = c0; base < c1; base += kTile) {
-    __syncthreads();
-    const int j = base + threadIdx.x;
-    if (j < c1) {
-      const float4 pj = d.pos[j];
-      cpos[threadIdx.x] = make_float4(pj.x, pj.y, pj.z, d.q[j]);
-      cb[threadIdx.x] = d.br_f[j];
+  float rA = 0.f, rx = 0.f, ry = 0.f, rz = 0.f, re = 0.f;
+  for (int c = 0; c < 4; c++) {
+    const int j0 = tp.y * 256 + c * 64 + lane;
+    float cx = -1e6f - (float)lane, cy = 0.f, cz = 0.f, cq = 0.f, cb = 1.f;
+    if (j0 < d.n) {
+      const float4 p = d.pos[j0];
+      cx = p.x; cy = p.y; cz = p.z;
+      cq = d.q[j0];
+      cb = d.br_f[j0];
     }
-    __syncthreads();
-    const int nc = min(kTile, c1 - base);
-    float tA = 0.f, tx = 0.f, ty = 0.f, tz = 0.f, te = 0.f;
-    for (int c = 0; c < nc; c++) {
-      const float4 pj = cpos[c];
-      const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+    int cj = j0;
+    float cA = 0.f, cgx = 0.f, cgy = 0.f, cgz = 0.f;
+    for (int s = 0; s < 64; s++) {
+      const float ex = cx - xi, ey = cy - yi, ez = cz - zi;
       const float d2 = ex * ex + ey * ey + ez * ez;
-      const bool valid = (base + c != i) && (d2 > 0.f);
-      const float qq = valid ? qi * pj.w : 0.f;
-      const float bb = bi * cb[c];
+      const bool valid = (cj != i) && (d2 > 0.f);
+      const float qq = valid ? qi * cq : 0.f;
+      const float bb = bi * cb;
       const float et = __expf(-0.25f * d2 * __builtin_amdgcn_rcpf(bb));
       const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
       const float fgb3 = fgb * fgb * fgb;
-      te += qq * fgb;
+      re += qq * fgb;
       const float mw = qq * (1.f - 0.25f * et) * fgb3;
-      tx += mw * ex;
-      ty += mw * ey;
-      tz += mw * ez;
-      tA += qq * (bb + 0.25f * d2) * et * fgb3;
+      const float at = qq * (bb + 0.25f * d2) * et * fgb3;
+      rx += mw * ex; ry += mw * ey; rz += mw * ez;
+      rA += at;
+      cA += at;
+      cgx -= mw * ex; cgy -= mw * ey; cgz -= mw * ez;
+      cx = wave_rot(cx); cy = wave_rot(cy); cz = wave_rot(cz); cq = wave_rot(cq); cb = wave_rot(cb);
+      cj = wave_rot(cj);
+      cA = wave_rot(cA); cgx = wave_rot(cgx); cgy = wave_rot(cgy); cgz = wave_rot(cgz);
     }
-    A += tA; gx += tx; gy += ty; gz += tz; e += te;
+    // 64 rotations bring every column's sums back to its starting lane
+    colacc[w][0][c * 64 + lane] = cA;
+    colacc[w][1][c * 64 + lane] = cgx;
+    colacc[w][2][c * 64 + lane] = cgy;
+    colacc[w][3][c * 64 + lane] = cgz;
   }
+  const double e = wave_sum((double)re * (diag ? 0.5 : 1.0));
+  if (lane == 0) ered[w] = e;
   if (i < d.n) {
-    double *p = d.gb_part + (size_t)chunk * 5 * d.n;
-    p[i] = A;
-    p[(size_t)d.n + i] = gx;
-    p[2 * (size_t)d.n + i] = gy;
-    p[3 * (size_t)d.n + i] = gz;
-    p[4 * (size_t)d.n + i] = e;
+    fix_add(&d.acc_gbA[i], (double)rA);
+    fix_add(&d.acc_gbg[3 * (size_t)i], (double)rx);
+    fix_add(&d.acc_gbg[3 * (size_t)i + 1], (double)ry);
+    fix_add(&d.acc_gbg[3 * (size_t)i + 2], (double)rz);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) d.gb_epart[blockIdx.x] = ered[0] + ered[1] + ered[2] + ered[3];
+  const int j = tp.y * 256 + threadIdx.x;
+  if (!diag && j < d.n) {
+    const int t = threadIdx.x;
+    fix_add(&d.acc_gbA[j], (double)colacc[0][0][t] + colacc[1][0][t] + colacc[2][0][t] + colacc[3][0][t]);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      fix_add(&d.acc_gbg[3 * (size_t)j + q],
+              (double)colacc[0][1 + q][t] + colacc[1][1 + q][t] + colacc[2][1 + q][t] + colacc[3][1 + q][t]);
   }
 }
 
 void launch_gbpair(const Dev &d, hipStream_t s) {
-  hipLaunchKernelGGL(k_gbpair, dim3((d.n + kRows - 1) / kRows, d.nchunk_gb), dim3(kRows), 0, s, d);
+  if (d.ntiles) hipLaunchKernelGGL(k_gbpair, dim3(d.ntiles), dim3(256), 0, s, d);
 }
 
 // -------------------------------------------- chain rule at constant sp

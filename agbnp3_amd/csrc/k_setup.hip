@@ -33,9 +33,12 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
 #pragma unroll
       for (int k = 0; k < TG_N; k++) d.acc_grad[(size_t)i * TG_N + k] = 0;
     }
-    d.acc_hb[3 * (size_t)i] = 0;
-    d.acc_hb[3 * (size_t)i + 1] = 0;
-    d.acc_hb[3 * (size_t)i + 2] = 0;
+    d.acc_gbA[i] = 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      d.acc_hb[3 * (size_t)i + q] = 0;
+      d.acc_gbg[3 * (size_t)i + q] = 0;
+    }
   }
   if (i == 0) {
     *d.status = 0;
