@@ -28,7 +28,14 @@ EXPORTED = (
     "agbnp3_set_options", "agbnp3_ener_device", "agbnp3_sync", "agbnp3_stream", "agbnp3_natoms",
     "agbnp3_get_int2ext", "agbnp3_get_near_list", "agbnp3_get_tables", "agbnp3_get_internal",
     "agbnp3_get_counts", "agbnp3_kernel_times", "agbnp3_q4_tables",
+    "agbnp3_get_tuning", "agbnp3_set_tuning", "agbnp3_tune",
+    "agbnp3_batch_new", "agbnp3_batch_delete", "agbnp3_batch_offsets", "agbnp3_batch_set_options",
+    "agbnp3_batch_ener_device", "agbnp3_batch_ener", "agbnp3_batch_sync", "agbnp3_batch_stream",
+    "agbnp3_batch_graph_capture", "agbnp3_batch_graph_launch", "agbnp3_batch_kernel_times",
+    "agbnp3_batch_get_tuning", "agbnp3_batch_set_tuning", "agbnp3_batch_tune",
+    "agbnp3_set_nblist_reuse", "agbnp3_batch_set_nblist_reuse", "agbnp3_nblist_rebuilds",
 )
+TUNING_PARAMS = ("rows", "desc_chunk", "chain_chunk", "desc_grid", "gb_grid", "chain_grid", "cap")
 NBLIST_EXPORTED = ("nblist_reset_neighbor_list", "nblist_delete_neighbor_list", "nblist_reallocate_neighbor_list")
 
 
@@ -87,6 +94,27 @@ def load():
         L.agbnp3_get_counts.argtypes = [ci, vp]
         L.agbnp3_kernel_times.argtypes = [ci, vp, ci, ctypes.c_char_p, ci]
         L.agbnp3_q4_tables.argtypes = [ci, vp, vp, vp, vp]
+        L.agbnp3_get_tuning.argtypes = [ci, vp, ci]
+        L.agbnp3_set_tuning.argtypes = [ci, vp, ci]
+        L.agbnp3_tune.argtypes = [ci, vp, vp, vp, ci, ctypes.c_char_p, ci]
+        L.agbnp3_batch_new.argtypes = [ctypes.POINTER(ci), ci, vp]
+        L.agbnp3_batch_delete.argtypes = [ci]
+        L.agbnp3_batch_offsets.argtypes = [ci, ctypes.POINTER(ci), vp]
+        L.agbnp3_batch_set_options.argtypes = [ci, ci]
+        L.agbnp3_batch_ener_device.argtypes = [ci] + [vp] * 11
+        L.agbnp3_batch_ener.argtypes = [ci, vp, vp, vp, vp]
+        L.agbnp3_batch_sync.argtypes = [ci]
+        L.agbnp3_batch_stream.restype = vp
+        L.agbnp3_batch_stream.argtypes = [ci]
+        L.agbnp3_batch_graph_capture.argtypes = [ci] + [vp] * 11
+        L.agbnp3_batch_graph_launch.argtypes = [ci]
+        L.agbnp3_batch_kernel_times.argtypes = [ci, vp, ci, ctypes.c_char_p, ci]
+        L.agbnp3_batch_get_tuning.argtypes = [ci, vp, ci]
+        L.agbnp3_batch_set_tuning.argtypes = [ci, vp, ci]
+        L.agbnp3_batch_tune.argtypes = [ci, vp, vp, vp, ci, ctypes.c_char_p, ci]
+        L.agbnp3_set_nblist_reuse.argtypes = [ci, ctypes.c_float, ci]
+        L.agbnp3_batch_set_nblist_reuse.argtypes = [ci, ctypes.c_float, ci]
+        L.agbnp3_nblist_rebuilds.argtypes = [ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]
         _nb.nblist_reallocate_neighbor_list.argtypes = [ctypes.POINTER(NeighList), ci, ci]
         _nb.nblist_reset_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
         _nb.nblist_delete_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
@@ -269,3 +297,117 @@ class Instance:
 
     def sync(self):
         return load().agbnp3_sync(self.tag)
+
+    def get_tuning(self):
+        p = np.zeros(len(TUNING_PARAMS), np.int32)
+        if load().agbnp3_get_tuning(self.tag, _p(p), p.size) != AGBNP_OK:
+            raise RuntimeError("agbnp3_get_tuning failed")
+        return dict(zip(TUNING_PARAMS, p.tolist()))
+
+    def set_tuning(self, **kw):
+        t = self.get_tuning()
+        t.update(kw)
+        p = np.array([t[k] for k in TUNING_PARAMS], np.int32)
+        if load().agbnp3_set_tuning(self.tag, _p(p), p.size) != AGBNP_OK:
+            raise RuntimeError("agbnp3_set_tuning failed")
+
+    def set_nblist_reuse(self, skin, every=0):
+        if load().agbnp3_set_nblist_reuse(self.tag, float(skin), int(every)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_set_nblist_reuse failed")
+
+    def nblist_rebuilds(self):
+        """(calls, candidate rebuilds) since set_nblist_reuse."""
+        c, r = ctypes.c_int(), ctypes.c_int()
+        if load().agbnp3_nblist_rebuilds(self.tag, ctypes.byref(c), ctypes.byref(r)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_nblist_rebuilds failed")
+        return c.value, r.value
+
+    def tune(self, x, y, z, reps=3):
+        """Autotune on device coordinates (pointers); returns the trial report."""
+        rep = ctypes.create_string_buffer(1 << 16)
+        if load().agbnp3_tune(self.tag, x, y, z, int(reps), rep, len(rep)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_tune failed")
+        return rep.value.decode()
+
+
+class Batch:
+    """agbnp3_batch_new() over a list of Instance objects."""
+
+    def __init__(self, instances):
+        L = load()
+        tags = np.array([i.tag for i in instances], np.int32)
+        b = ctypes.c_int(-1)
+        if L.agbnp3_batch_new(ctypes.byref(b), tags.size, _p(tags)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_new failed")
+        self.id = b.value
+        self.nsys = tags.size
+        off = np.zeros(self.nsys + 1, np.int32)
+        L.agbnp3_batch_offsets(self.id, None, _p(off))
+        self.offsets = off
+        self.n = int(off[-1])
+
+    def set_options(self, options):
+        if load().agbnp3_batch_set_options(self.id, int(options)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_set_options failed")
+
+    def ener(self, x, y, z):
+        """Host arrays (concatenated caller order); returns a dict like agbnp3_ener's, energies [nsys, 7]."""
+        x, y, z = (np.ascontiguousarray(a, np.float64) for a in (x, y, z))
+        n = self.n
+        out = np.zeros(15 * n + 7 * self.nsys)
+        if load().agbnp3_batch_ener(self.id, _p(x), _p(y), _p(z), _p(out)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_ener failed")
+        return dict(sp=out[:n], br=out[n:2 * n], surf_area=out[2 * n:3 * n],
+                    dgbdr=out[3 * n:6 * n].reshape(n, 3), dvwdr=out[6 * n:9 * n].reshape(n, 3),
+                    decav=out[9 * n:12 * n].reshape(n, 3), dehb=out[12 * n:15 * n].reshape(n, 3),
+                    energies=out[15 * n:].reshape(self.nsys, 7))
+
+    def ener_device(self, *ptrs):
+        return load().agbnp3_batch_ener_device(self.id, *ptrs)
+
+    def graph_capture(self, *ptrs):
+        if load().agbnp3_batch_graph_capture(self.id, *ptrs) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_graph_capture failed")
+
+    def graph_launch(self):
+        return load().agbnp3_batch_graph_launch(self.id)
+
+    def sync(self):
+        return load().agbnp3_batch_sync(self.id)
+
+    def stream(self):
+        return load().agbnp3_batch_stream(self.id)
+
+    def kernel_times(self):
+        ms = np.zeros(64, np.float32)
+        names = ctypes.create_string_buffer(4096)
+        k = load().agbnp3_batch_kernel_times(self.id, _p(ms), 64, names, 4096)
+        return dict(zip(names.value.decode().split(";"), ms[:k].tolist())) if k > 0 else {}
+
+    def get_tuning(self):
+        p = np.zeros(len(TUNING_PARAMS), np.int32)
+        if load().agbnp3_batch_get_tuning(self.id, _p(p), p.size) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_get_tuning failed")
+        return dict(zip(TUNING_PARAMS, p.tolist()))
+
+    def set_tuning(self, **kw):
+        t = self.get_tuning()
+        t.update(kw)
+        p = np.array([t[k] for k in TUNING_PARAMS], np.int32)
+        if load().agbnp3_batch_set_tuning(self.id, _p(p), p.size) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_set_tuning failed")
+
+    def set_nblist_reuse(self, skin, every=0):
+        if load().agbnp3_batch_set_nblist_reuse(self.id, float(skin), int(every)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_set_nblist_reuse failed")
+
+    def tune(self, x, y, z, reps=3):
+        rep = ctypes.create_string_buffer(1 << 16)
+        if load().agbnp3_batch_tune(self.id, x, y, z, int(reps), rep, len(rep)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_tune failed")
+        return rep.value.decode()
+
+    def delete(self):
+        if self.id is not None:
+            load().agbnp3_batch_delete(self.id)
+            self.id = None

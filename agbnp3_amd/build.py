@@ -1,6 +1,8 @@
 """Builds every native library of the package in-tree.
 
-  lib/libagbnp3.so        the engine (HIP for gfx950 + host C-ABI)
+  lib/libagbnp3.so        the engine (HIP for gfx950 + host C-ABI), float_i = double
+  lib/libagbnp3_sp.so     the same engine behind the single-precision interface
+                          (agbnp3.h built with SINGLE_PREC_INTERFACE: float_i = float)
   lib/libnblist.so        NeighList container (C)
   lib/libagbnp3_synth.so  seeded synthetic solutes (C)
 and, as test infrastructure, oracle/liboracle.so plus the reference build
@@ -23,6 +25,14 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
 ]
+# Translation units whose kernels must reproduce the reference's FP32
+# distance tests exactly: contraction is allowed only where the source permits
+# it, so `#pragma clang fp contract(off)` is honoured (the default "fast" mode
+# fuses across statements and ignores the pragma).
+HIP_FILE_FLAGS = {
+    "k_setup.hip": ["-ffp-contract=on"],
+    "k_hb.hip": ["-ffp-contract=on"],
+}
 
 
 def _run(cmd, verbose):
@@ -55,13 +65,20 @@ def build_hip(verbose=True, jobs=6):
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC] + HIP_FLAGS + inc + ["-c", os.path.join(_paths.CSRC_DIR, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + HIP_FILE_FLAGS.get(src, []) + inc +
+                    ["-c", os.path.join(_paths.CSRC_DIR, src), "-o", obj])
+    # the host C-ABI once more with float_i = float
+    host_sp = os.path.join(objdir, "agbnp3_host_sp.o")
+    cmds.append([HIPCC] + HIP_FLAGS + ["-DSINGLE_PREC_INTERFACE"] + inc +
+                ["-c", os.path.join(_paths.CSRC_DIR, "agbnp3_host.hip"), "-o", host_sp])
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
-    tmp = _paths.LIBAGBNP3 + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs +
-         [f"-L{_paths.LIB_DIR}", "-lnblist", "-Wl,-rpath,$ORIGIN"], verbose)
-    os.replace(tmp, _paths.LIBAGBNP3)
+    kernels = objs[1:]
+    for lib, host in ((_paths.LIBAGBNP3, objs[0]), (_paths.LIBAGBNP3_SP, host_sp)):
+        tmp = lib + ".tmp"
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, host] + kernels +
+             [f"-L{_paths.LIB_DIR}", "-lnblist", "-Wl,-rpath,$ORIGIN"], verbose)
+        os.replace(tmp, lib)
 
 
 def build_oracle(verbose=True):

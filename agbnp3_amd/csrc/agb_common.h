@@ -1,9 +1,11 @@
-// Shared definitions of the gfx950 AGBNP3 engine: model constants, the
-// per-instance device view passed by value to every kernel, and the launcher
+// Shared definitions of the gfx950 AGBNP3 engine: model constants, the device
+// view of a pack of systems passed by value to every kernel, and the launcher
 // prototypes implemented by the kernel translation units.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 namespace agb {
 
@@ -33,7 +35,11 @@ enum : int {
   ST_TREE_OVERFLOW = 2,     // an overlap group exceeded its LDS capacity
   ST_WS_OVERFLOW = 4,       // a water site had more candidates than its buffer
   ST_GRID_OVERFLOW = 8,     // cell grid larger than allocated
+  ST_CAND_OVERFLOW = 16,    // a kept neighbour-candidate row exceeded kCandCap
 };
+
+// kept neighbour candidates per heavy row (neighbour-list reuse)
+constexpr int kCandCap = 256;
 
 // energy partial-sum slots (per block, reduced in a fixed order)
 enum : int {
@@ -50,14 +56,14 @@ struct WSite {
   int parent[4];
   int nparents;
   int type;
-  int stale0;        // atom whose spe the reference reads first (quirk Q3)
-  int pad;
+  int stale0;        // atom whose spe the reference reads first (quirk Q3); -1: none (0)
+  int sys;           // system of the site
   float dpos[4][3][3];   // d pos_i / d x_{parent p, j}
 };
 
 // static description of the water sites of one HB-active atom
 struct WsDef {
-  int atom;          // internal index of the acceptor / polar hydrogen
+  int atom;          // pack index of the acceptor / polar hydrogen
   int kind;          // placement geometry (WS_* below)
   int nsite;         // sites this atom creates
   int first;         // index of its first site
@@ -73,20 +79,38 @@ struct GridParams {
   int ncell;
 };
 
-// Device view of one instance. Internal order: heavy atoms first.
+// cells per system grid (LDS histogram of the binning kernel)
+constexpr int kMaxCells = 8192;
+
+// One system (an agbnp3 instance) inside a pack.  A pack concatenates the
+// per-atom arrays of its systems; inside each system heavy atoms come first.
+struct SysInfo {
+  int a0, n, nh;        // atoms [a0, a0 + n), heavy atoms [a0, a0 + nh)
+  int h0;               // its first heavy row (rows number the pack's heavy atoms)
+  int ws0, nws;         // water sites [ws0, ws0 + nws)
+  int blk0, nblk;       // atom blocks [blk0, blk0 + nblk) (energy partials)
+  int tile0, ntile;     // GB tile-pair energy slots [tile0, tile0 + ntile)
+  float df;             // dielectric factor -0.5 (1/eps_in - 1/eps_out), FP32
+  float rmax_heavy;     // largest heavy-atom R (cell size)
+  double df64;          // same factor in FP64
+};
+
+// work-queue heads (cleared by the ingest kernel of every call)
+enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_COUNT };
+
+// Device view of a pack of one or more systems, passed by value to every kernel.
 struct Dev {
-  int n, nh;             // atoms, heavy atoms
-  int ntypes;
+  int n, nh;             // atoms, heavy rows of the pack
+  int nsys;
+  int ntypes;            // radius types (union over the systems)
   int cap;               // neighbour-list row capacity
-  int ncell_max;
   int nws, nwsdef;       // water sites, HB-active atoms
-  int ws_cap;            // candidate capacity per water site
   int options;
-  float eps_df;          // dielectric factor -0.5 (1/ein - 1/eout), FP32
-  double df;             // same in FP64
-  float rmax_heavy;      // largest heavy-atom R
-  // static inputs
-  const int *int2ext;
+  const SysInfo *sys;
+  const int *asys;       // system of each atom
+  const int *hrow;       // atom of each heavy row
+  // static inputs (pack order)
+  const int *int2ext;    // pack order -> caller order (concatenated per system)
   const float *R;        // r + 0.5
   const float *q;
   const int *rtype;
@@ -95,66 +119,85 @@ struct Dev {
   const double *vols;    // 4/3 pi R^3
   const float2 *tbl;     // [ntypes*ntypes][512] (y, y2)
   float tdx, tdxinv;
-  const int *row_perm;   // type-grouped row order, -1 padded
-  const int *block_type; // row type of each 256-row block
-  const int *block_hrows;  // 1 if every row of the block is a hydrogen
+  // row blocks of the spline kernels: rows of one radius type and one system
+  int rows;              // rows per block (64, 128 or 256)
   int nrow_blocks;
+  const int *row_perm;   // [nrow_blocks][rows] pack atom or -1
+  const int4 *rblk;      // per block: radius type, system, hydrogen rows only, -
+  // work items, sorted by decreasing size, taken from the queues by
+  // persistent blocks: descreening / chain (row block, column range), GB
+  // tile pairs (row range, column range; diagonal when they start together)
+  const int4 *desc_items, *chain_items, *gb_items;
+  int ndesc_items, nchain_items, ngb_items;
+  const int *gb_eslot;   // energy slot of each GB item
+  unsigned *queue;       // [Q_COUNT]
+  const int2 *ablk;      // atom blocks: [start, end) inside one system
+  int nablk;
   const WsDef *wsdef;
   const int *ws_stale0;  // per site: atom whose spe the first subtraction reads
   // per-call state
   float4 *pos;           // x, y, z, R
-  float *bbox_part;      // [6][nblk]
-  GridParams *grid;
-  int *cell_count, *cell_start, *cell_atoms;
-  int *nne, *nbr;
+  GridParams *grid;      // [nsys]
+  int *cell_start;       // [nsys][kMaxCells + 1]
+  int *cell_atoms, *cell_tmp;          // [n]: heavy atoms by cell, per system range
+  int *nne, *nbr;                      // [nh], [nh][cap]
+  // neighbour-list reuse (skin > 0): candidates within cutoff + skin, the
+  // heavy-atom positions they were built from, and [0] candidates valid,
+  // [1] calls, [2] rebuild this call, [3] rebuilds so far
+  float skin;
+  int reuse_every;                     // forced rebuild every this many calls (0: guard only)
+  int *cand_nne, *cand_nbr;            // [nh], [nh][kCandCap]
+  float4 *ref_pos;                     // [n]
+  int *nbl_state;
   int *status;
-  long long *acc_vol, *acc_sa;         // tree pass A [nh]
-  long long *acc_grad;                 // tree pass B [nh][TG_N]
+  long long *acc_vol, *acc_sa;         // tree pass A [n]
+  long long *acc_grad;                 // tree pass B [n][TG_N]
   long long *acc_hb;                   // HB direct gradient [n][3]
-  long long *acc_derh;                 // [nh]
+  long long *acc_derh;                 // [n]
+  long long *acc_beta;                 // descreening sums [n]
+  long long *acc_gbA, *acc_gbg;        // GB pairs: A_i [n], gradient [n][3]
+  long long *acc_ch;                   // chain rule [n][8]: gb xyz, vdw xyz, U, V
   unsigned long long *counters;        // [0] kept overlaps [1] evaluated [2] nbl entries [3] max row
   double *volumep_raw, *sa_raw, *sa_f, *sp, *spe, *psvol, *gammap;   // [n]
   float *sp_f, *spe_f;                 // FP32 copies for pair kernels
-  double *beta_part;                   // [nchunk_desc][n]
-  int nchunk_desc;
   double *beta, *br, *brw, *swf;       // [n]
   float *br_f;
-  long long *acc_gbA, *acc_gbg;        // GB pairs: A_i [n], gradient [n][3] (fixed point)
-  double *gb_epart;                    // per tile pair: sum of q_i q_j / f
-  const int2 *tile_ij;                 // tile pairs (I <= J) of 256-atom blocks
-  int ntiles;
+  double *gb_epart;                    // per GB energy slot: sum of q_i q_j / f
   double *A, *q2ab, *abrw;             // [n]
   float *q2ab_f, *abrw_f;
-  double *ch_part;                     // [nchunk_ch][8][n]: gbx,gby,gbz,vwx,vwy,vwz,U,V
-  int nchunk_ch;
   double *deru, *derv, *derus, *dervs, *derh;   // [n]
   float *deru_f, *derv_f, *derus_f, *dervs_f, *derh_f, *gammap_f;
   WSite *ws;
   double *ws_part;                     // per-site energy khb*s
-  double *epart;                       // [EP_COUNT][epart_stride]
-  int epart_stride;
+  double *epart;                       // [EP_COUNT][nablk]
+};
+
+// launch shapes chosen per pack (the autotuner may change them)
+struct Tuning {
+  int rows = 256;          // rows per spline-kernel block
+  int desc_chunk = 0;      // columns per descreening item (0: whole system)
+  int chain_chunk = 0;     // columns per chain item (0: whole system)
+  int desc_grid = 0;       // persistent blocks (0: one per item)
+  int gb_grid = 0;
+  int chain_grid = 0;
 };
 
 // ---- launchers (kernel translation units) ------------------------------
 void launch_ingest(const Dev &d, const double *x, const double *y, const double *z, hipStream_t s);
-void launch_grid(const Dev &d, hipStream_t s);
 void launch_cells(const Dev &d, hipStream_t s);
 void launch_nblist(const Dev &d, hipStream_t s);
 void launch_tree(const Dev &d, int pass, hipStream_t s);
 void launch_scale(const Dev &d, hipStream_t s);
-void launch_descreen(const Dev &d, hipStream_t s);
+void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s);
 void launch_born(const Dev &d, hipStream_t s);
-void launch_gbpair(const Dev &d, hipStream_t s);
+void launch_gbpair(const Dev &d, const Tuning &t, hipStream_t s);
 void launch_q2ab(const Dev &d, hipStream_t s);
-void launch_chain(const Dev &d, hipStream_t s);
+void launch_chain(const Dev &d, const Tuning &t, hipStream_t s);
 void launch_uv(const Dev &d, hipStream_t s);
+hipError_t prepare_pairs(int ntypes);
 void launch_ws_place(const Dev &d, hipStream_t s);
 void launch_ws_free(const Dev &d, hipStream_t s);
 void launch_final(const Dev &d, double *sp, double *br, double *sa, double *dgbdr, double *dvwdr,
                   double *decav, double *dehb, double *energies, hipStream_t s);
-
-// block counts used by the energy partial sums
-int scale_blocks(const Dev &d);
-int born_blocks(const Dev &d);
 
 }  // namespace agb

@@ -39,6 +39,17 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
+// ------------------------------------------------------------ work queues
+// Persistent blocks take work items from a queue head in memory; every block
+// leaves once the head has passed the item count, so the grid always drains.
+// `slot` is block-shared scratch for the broadcast.
+__device__ __forceinline__ int next_item(unsigned *head, int *slot) {
+  __syncthreads();                       // the previous item is finished by all waves
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(head, 1u);
+  __syncthreads();
+  return *slot;
+}
+
 // ------------------------------------------------------------------ cells
 __device__ __forceinline__ void cell_coords(const GridParams &g, float x, float y, float z, int &cx, int &cy,
                                             int &cz) {

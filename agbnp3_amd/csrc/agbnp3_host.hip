@@ -1,6 +1,8 @@
 // Host side of libagbnp3: the agbnp3.h C API (instance registry, agbnp3_new /
 // agbnp3_ener / agbnp3_delete), the Q4 spline tables, the static water-site
-// topology, device arenas and the per-call kernel pipeline on one stream.
+// topology, packs (the device arena of one or more systems, their work
+// items and launch shapes), the per-call kernel pipeline on one stream,
+// batches of instances with hipGraph replay, and the launch-shape autotuner.
 #include <float.h>
 #include <math.h>
 #include <stdarg.h>
@@ -9,8 +11,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
+#include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "agb_common.h"
@@ -136,11 +141,11 @@ void build_table(float b, float *y, float *y2, float &dx, float &dxinv) {
   dxinv = 1. / da;
 }
 
-// ------------------------------------------------------------- instance
-struct Instance {
+// ------------------------------------------------------------- model
+// Host copy of one system's parameters, in internal order (heavy atoms first)
+struct Model {
   int natoms = 0, nheavy = 0;
   int verbose = 0;
-  int options = 0;
   int ws_error_atom = -1;              // TRIGONAL_OOP topology error (reported by ener)
   std::vector<int> int2ext, ext2int;
   std::vector<float> R, q, igamma, sgamma, ialpha, salpha, hbcorr, galpha;
@@ -151,27 +156,55 @@ struct Instance {
   std::vector<float> radii, ty, ty2;
   float tdx = 0, tdxinv = 0;
   float eps_in = 1, eps_out = 1;
-  std::vector<int> row_perm, block_type, block_hrows;
   std::vector<WsDef> wsdef;
-  std::vector<int> ws_stale0;
+  std::vector<int> ws_stale0;          // per site; -1: no atom (spe 0)
   int nws = 0;
-  // device
+};
+
+// ------------------------------------------------------------- pack
+// Device arena, work items and launch shapes of one or more systems that
+// are evaluated together.
+struct Pack {
+  int n = 0, nh = 0, nsys = 0, ntypes = 0;
+  int options = 0;
+  int cap = 128;                       // neighbour-list row capacity (64, 128, 256)
+  float skin = 0.f;                    // neighbour-list reuse (0: rebuild every call)
+  int reuse_every = 0;
+  Tuning tune;
+  std::vector<float> radii;            // radius types (union over the systems)
+  std::vector<SysInfo> sinfo;
+  std::vector<int> rtype;              // per atom
+  std::vector<char> heavy;             // per atom
   hipStream_t stream = nullptr;
-  std::vector<void *> allocs;
+  std::vector<void *> allocs, item_allocs;
   Dev d{};
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;      // coordinate staging (device)
-  double *dout = nullptr;                                  // outputs: 3n + 12n + 7
+  double *dout = nullptr;                                  // outputs: 15n + 7 nsys
   double *hstage = nullptr;                                // pinned host staging
-  size_t hstage_len = 0;
   // timing mode: one event set per recorded call, averaged when read back
   std::vector<std::vector<hipEvent_t>> ev_sets;
   std::vector<std::string> ev_names;
   int ncalls_timed = 0;
+  // captured pipeline
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+struct Instance {
+  Model m;
+  Pack p;
+};
+
+struct Batch {
+  std::vector<int> tags;
+  std::vector<int> ws_error;           // external atom of a TRIGONAL_OOP error per system, or -1
+  Pack p;
 };
 
 std::mutex g_mutex;
 bool g_initialized = false;
 std::vector<Instance *> g_slots;
+std::vector<Batch *> g_batches;
 
 Instance *get_instance(int tag, const char *who) {
   if (!g_initialized) {
@@ -185,48 +218,72 @@ Instance *get_instance(int tag, const char *who) {
   return g_slots[tag];
 }
 
+Batch *get_batch(int id, const char *who) {
+  if (!g_initialized) {
+    errprint("%s: agbnp library is not initialized.\n", who);
+    return nullptr;
+  }
+  if (id < 0 || id >= (int)g_batches.size() || !g_batches[id]) {
+    errprint("%s: invalid batch %d.\n", who, id);
+    return nullptr;
+  }
+  return g_batches[id];
+}
+
 // device allocation helpers: 0 on success, non-zero on failure
 template <typename T>
-int dalloc(Instance *I, T **p, size_t count) {
+int dalloc(std::vector<void *> &list, T **p, size_t count) {
   void *q = nullptr;
   if (count == 0) count = 1;
   hipError_t e = hipMalloc(&q, count * sizeof(T));
   if (e != hipSuccess) {
-    errprint("agbnp3_new(): device allocation of %zu bytes failed: %s\n", count * sizeof(T), hipGetErrorString(e));
+    errprint("agbnp3: device allocation of %zu bytes failed: %s\n", count * sizeof(T), hipGetErrorString(e));
     return 1;
   }
-  I->allocs.push_back(q);
+  list.push_back(q);
   *p = (T *)q;
   return 0;
 }
 
 template <typename T>
-int upload(Instance *I, const T **dst, const std::vector<T> &src) {
+int upload(std::vector<void *> &list, const T **dst, const std::vector<T> &src) {
   T *p = nullptr;
-  if (dalloc(I, &p, src.size())) return 1;
+  if (dalloc(list, &p, src.size())) return 1;
   if (!src.empty() && hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
-    errprint("agbnp3_new(): upload of %zu bytes failed\n", src.size() * sizeof(T));
+    errprint("agbnp3: upload of %zu bytes failed\n", src.size() * sizeof(T));
     return 1;
   }
   *dst = p;
   return 0;
 }
 
-void free_instance(Instance *I) {
-  if (!I) return;
-  if (I->stream) (void)hipStreamSynchronize(I->stream);
-  for (void *p : I->allocs) (void)hipFree(p);
-  if (I->hstage) (void)hipHostFree(I->hstage);
-  for (auto &set : I->ev_sets)
+void drop_graph(Pack &P) {
+  if (P.gexec) (void)hipGraphExecDestroy(P.gexec);
+  if (P.graph) (void)hipGraphDestroy(P.graph);
+  P.gexec = nullptr;
+  P.graph = nullptr;
+}
+
+void free_pack(Pack &P) {
+  if (P.stream) (void)hipStreamSynchronize(P.stream);
+  drop_graph(P);
+  for (void *p : P.allocs) (void)hipFree(p);
+  for (void *p : P.item_allocs) (void)hipFree(p);
+  P.allocs.clear();
+  P.item_allocs.clear();
+  if (P.hstage) (void)hipHostFree(P.hstage);
+  P.hstage = nullptr;
+  for (auto &set : P.ev_sets)
     for (hipEvent_t e : set) (void)hipEventDestroy(e);
-  if (I->stream) (void)hipStreamDestroy(I->stream);
-  delete I;
+  P.ev_sets.clear();
+  if (P.stream) (void)hipStreamDestroy(P.stream);
+  P.stream = nullptr;
 }
 
 // static water-site topology: geometry per HB-active atom, following the
 // dispatch of agbnp3_create_ws_ofatom (agbnp3.c:1920-2012); atoms whose
 // topology does not fit get no sites, as in the reference
-void build_water_sites(Instance *I) {
+void build_water_sites(Model *I) {
   const int n = I->natoms, nh = I->nheavy;
   I->wsdef.clear();
   I->ws_stale0.clear();
@@ -304,131 +361,234 @@ void build_water_sites(Instance *I) {
   // site s reads the last parent of site s-1 first; site 0 the index after
   // the heavy atoms (a hydrogen, spe = 0)
   std::vector<int> st(nsite);
-  for (int s = 0; s < nsite; s++) st[s] = s == 0 ? nh : I->ws_stale0[s - 1];
+  for (int s = 0; s < nsite; s++) st[s] = s == 0 ? (nh < n ? nh : -1) : I->ws_stale0[s - 1];
   I->ws_stale0 = st;
   I->nws = nsite;
 }
 
-int choose_chunks(int rowblocks, int ncols) {
-  int c = (1024 + rowblocks - 1) / std::max(rowblocks, 1);
-  c = std::max(1, std::min(c, std::max(1, ncols / 512)));
-  return c;
-}
-
-int setup_device(Instance *I) {
-  const int n = I->natoms, nh = I->nheavy;
-  Dev &d = I->d;
-  d = Dev{};
-  d.n = n;
-  d.nh = nh;
-  d.ntypes = I->ntypes;
-  d.cap = 128;
-  d.ncell_max = 8192;
-  d.options = I->options;
-  d.df = -0.5 * (1.0 / (double)I->eps_in - 1.0 / (double)I->eps_out);
-  d.eps_df = (float)d.df;
-  float rmax = 0.f;
-  for (int i = 0; i < nh; i++) rmax = std::max(rmax, I->R[i]);
-  d.rmax_heavy = rmax;
-  if (upload(I, &d.int2ext, I->int2ext) || upload(I, &d.R, I->R) || upload(I, &d.q, I->q) ||
-      upload(I, &d.rtype, I->rtype) || upload(I, &d.galpha, I->galpha) || upload(I, &d.igamma, I->igamma) ||
-      upload(I, &d.sgamma, I->sgamma) || upload(I, &d.ialpha, I->ialpha) || upload(I, &d.salpha, I->salpha) ||
-      upload(I, &d.hbcorr_f, I->hbcorr) || upload(I, &d.vols, I->vols))
-    return AGBNP_ERR;
-  {
-    std::vector<float2> tb(I->ty.size());
-    for (size_t k = 0; k < tb.size(); k++) tb[k] = make_float2(I->ty[k], I->ty2[k]);
-    if (upload(I, &d.tbl, tb)) return AGBNP_ERR;
-    d.tdx = I->tdx;
-    d.tdxinv = I->tdxinv;
-  }
-  // rows grouped by radius type, 256-row blocks of a single type
-  I->row_perm.clear();
-  I->block_type.clear();
-  I->block_hrows.clear();
-  for (int t = 0; t < I->ntypes; t++) {
-    std::vector<int> rows;
-    for (int i = 0; i < n; i++)
-      if (I->rtype[i] == t) rows.push_back(i);
-    for (size_t b = 0; b < rows.size(); b += 256) {
-      int allh = 1;
-      for (size_t k = 0; k < 256; k++) {
-        const int r = b + k < rows.size() ? rows[b + k] : -1;
-        I->row_perm.push_back(r);
-        if (r >= 0 && r < nh) allh = 0;
+// --------------------------------------------------------- work items
+// Row blocks of one radius type and one system; descreening / chain items
+// (row block x column range) and GB tile pairs, each list sorted by
+// decreasing size so persistent blocks take the large items first.
+int build_items(Pack &P) {
+  for (void *p : P.item_allocs) (void)hipFree(p);
+  P.item_allocs.clear();
+  drop_graph(P);
+  Dev &d = P.d;
+  const int rows = P.tune.rows;
+  std::vector<int> row_perm;
+  std::vector<int4> rblk;
+  for (int b = 0; b < P.nsys; b++) {
+    const SysInfo &S = P.sinfo[b];
+    for (int t = 0; t < P.ntypes; t++) {
+      std::vector<int> r;
+      for (int i = S.a0; i < S.a0 + S.n; i++)
+        if (P.rtype[i] == t) r.push_back(i);
+      for (size_t k = 0; k < r.size(); k += rows) {
+        int allh = 1;
+        for (int q = 0; q < rows; q++) {
+          const int a = k + q < r.size() ? r[k + q] : -1;
+          row_perm.push_back(a);
+          if (a >= 0 && P.heavy[a]) allh = 0;
+        }
+        rblk.push_back(make_int4(t, b, allh, 0));
       }
-      I->block_type.push_back(t);
-      I->block_hrows.push_back(allh);
     }
   }
-  d.nrow_blocks = (int)I->block_type.size();
-  if (upload(I, &d.row_perm, I->row_perm) || upload(I, &d.block_type, I->block_type) ||
-      upload(I, &d.block_hrows, I->block_hrows))
-    return AGBNP_ERR;
-  d.nwsdef = (int)I->wsdef.size();
-  d.nws = I->nws;
-  d.ws_cap = 128;
-  if (upload(I, &d.wsdef, I->wsdef) || upload(I, &d.ws_stale0, I->ws_stale0)) return AGBNP_ERR;
-  // per-call buffers
-  const int nblk = (n + 255) / 256;
-  d.nchunk_desc = choose_chunks(d.nrow_blocks, nh);
-  {
-    std::vector<int2> tiles;
-    const int nb = (n + 255) / 256;
-    for (int a = 0; a < nb; a++)
-      for (int b = a; b < nb; b++) tiles.push_back(make_int2(a, b));
-    d.ntiles = (int)tiles.size();
-    if (upload(I, &d.tile_ij, tiles)) return AGBNP_ERR;
+  // column ranges
+  auto chunked = [&](int chunk_param, bool chain) {
+    std::vector<int4> items;
+    double total = 0.0;
+    for (size_t k = 0; k < rblk.size(); k++) {
+      const SysInfo &S = P.sinfo[rblk[k].y];
+      total += (double)rows * (chain ? (rblk[k].z ? S.nh : S.n) : S.nh);
+    }
+    int L = chunk_param;
+    if (L <= 0) {       // about 2048 items in all, at least 512 columns each
+      L = (int)std::ceil(total / (2048.0 * rows) / 256.0) * 256;
+      L = std::max(L, 512);
+    }
+    for (size_t k = 0; k < rblk.size(); k++) {
+      const SysInfo &S = P.sinfo[rblk[k].y];
+      const int c0 = S.a0, c1 = S.a0 + (chain ? (rblk[k].z ? S.nh : S.n) : S.nh);
+      for (int c = c0; c < c1; c += L) items.push_back(make_int4((int)k, c, std::min(c1, c + L), 0));
+    }
+    std::stable_sort(items.begin(), items.end(), [](const int4 &a, const int4 &b) { return a.z - a.y > b.z - b.y; });
+    return items;
+  };
+  std::vector<int4> desc = chunked(P.tune.desc_chunk, false);
+  std::vector<int4> chain = chunked(P.tune.chain_chunk, true);
+  // GB tile pairs of 256-atom tiles, I <= J
+  std::vector<std::pair<int4, int>> gb;
+  for (int b = 0; b < P.nsys; b++) {
+    const SysInfo &S = P.sinfo[b];
+    const int nb = (S.n + 255) / 256;
+    int k = 0;
+    for (int I = 0; I < nb; I++)
+      for (int J = I; J < nb; J++, k++) {
+        const int r0 = S.a0 + 256 * I, c0 = S.a0 + 256 * J;
+        gb.push_back({make_int4(r0, std::min(S.a0 + S.n, r0 + 256), c0, std::min(S.a0 + S.n, c0 + 256)), S.tile0 + k});
+      }
   }
-  d.nchunk_ch = choose_chunks(d.nrow_blocks, n);
-  d.epart_stride = nblk;
-  const size_t nn = n > 0 ? n : 1, nhh = nh > 0 ? nh : 1;
-  if (dalloc(I, &d.pos, nn) || dalloc(I, &d.bbox_part, 6 * (size_t)nblk) || dalloc(I, &d.grid, 1) ||
-      dalloc(I, &d.cell_start, d.ncell_max + 2 + nhh) || dalloc(I, &d.cell_atoms, nhh) ||
-      dalloc(I, &d.nne, nhh) || dalloc(I, &d.nbr, nhh * 256) || dalloc(I, &d.status, 1) ||
-      dalloc(I, &d.acc_vol, nhh) || dalloc(I, &d.acc_sa, nhh) || dalloc(I, &d.acc_grad, nhh * TG_N) ||
-      dalloc(I, &d.acc_hb, 3 * nn) || dalloc(I, &d.acc_derh, nhh) || dalloc(I, &d.counters, 4) ||
-      dalloc(I, &d.volumep_raw, nn) || dalloc(I, &d.sa_raw, nn) || dalloc(I, &d.sa_f, nn) ||
-      dalloc(I, &d.sp, nn) || dalloc(I, &d.spe, nn) || dalloc(I, &d.psvol, nn) || dalloc(I, &d.gammap, nn) ||
-      dalloc(I, &d.sp_f, nn) || dalloc(I, &d.spe_f, nn) ||
-      dalloc(I, &d.beta_part, (size_t)d.nchunk_desc * nn) || dalloc(I, &d.beta, nn) || dalloc(I, &d.br, nn) ||
-      dalloc(I, &d.brw, nn) || dalloc(I, &d.swf, nn) || dalloc(I, &d.br_f, nn) ||
-      dalloc(I, &d.acc_gbA, nn) || dalloc(I, &d.acc_gbg, 3 * nn) ||
-      dalloc(I, &d.gb_epart, (size_t)std::max(1, d.ntiles)) || dalloc(I, &d.A, nn) || dalloc(I, &d.q2ab, nn) ||
-      dalloc(I, &d.abrw, nn) || dalloc(I, &d.q2ab_f, nn) || dalloc(I, &d.abrw_f, nn) ||
-      dalloc(I, &d.ch_part, (size_t)d.nchunk_ch * 8 * nn) || dalloc(I, &d.deru, nn) || dalloc(I, &d.derv, nn) ||
-      dalloc(I, &d.derus, nn) || dalloc(I, &d.dervs, nn) || dalloc(I, &d.derh, nn) ||
-      dalloc(I, &d.deru_f, nn) || dalloc(I, &d.derv_f, nn) || dalloc(I, &d.derus_f, nn) ||
-      dalloc(I, &d.dervs_f, nn) || dalloc(I, &d.derh_f, nn) || dalloc(I, &d.gammap_f, nn) ||
-      dalloc(I, &d.ws, (size_t)std::max(1, I->nws)) || dalloc(I, &d.ws_part, (size_t)std::max(1, I->nws)) ||
-      dalloc(I, &d.epart, (size_t)EP_COUNT * nblk) || dalloc(I, &I->dx, nn) || dalloc(I, &I->dy, nn) ||
-      dalloc(I, &I->dz, nn) || dalloc(I, &I->dout, 15 * nn + 7))
+  std::stable_sort(gb.begin(), gb.end(), [](const std::pair<int4, int> &a, const std::pair<int4, int> &b) {
+    return (long long)(a.first.y - a.first.x) * (a.first.w - a.first.z) >
+           (long long)(b.first.y - b.first.x) * (b.first.w - b.first.z);
+  });
+  std::vector<int4> gbi(gb.size());
+  std::vector<int> gbs(gb.size());
+  for (size_t k = 0; k < gb.size(); k++) {
+    gbi[k] = gb[k].first;
+    gbs[k] = gb[k].second;
+  }
+  if (upload(P.item_allocs, &d.row_perm, row_perm) || upload(P.item_allocs, &d.rblk, rblk) ||
+      upload(P.item_allocs, &d.desc_items, desc) || upload(P.item_allocs, &d.chain_items, chain) ||
+      upload(P.item_allocs, &d.gb_items, gbi) || upload(P.item_allocs, &d.gb_eslot, gbs))
     return AGBNP_ERR;
-  d.cell_count = nullptr;
-  HIPCHK(hipMemset(d.epart, 0, sizeof(double) * EP_COUNT * nblk));
-  HIPCHK(hipMemset(d.ws_part, 0, sizeof(double) * std::max(1, I->nws)));
-  I->hstage_len = 15 * nn + 7 + 3 * nn;
-  HIPCHK(hipHostMalloc((void **)&I->hstage, I->hstage_len * sizeof(double), hipHostMallocDefault));
-  HIPCHK(hipStreamCreateWithFlags(&I->stream, hipStreamNonBlocking));
+  d.rows = rows;
+  d.nrow_blocks = (int)rblk.size();
+  d.ndesc_items = (int)desc.size();
+  d.nchain_items = (int)chain.size();
+  d.ngb_items = (int)gbi.size();
   return AGBNP_OK;
 }
 
-// ---------------------------------------------------------- the pipeline
-struct Step {
-  const char *name;
-};
+// Pack the systems `ms` (internal order each) into one device arena.
+int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
+  P.nsys = (int)ms.size();
+  P.options = options;
+  // radius types: union over the systems, by exact equality
+  std::vector<std::vector<int>> lmap(P.nsys);
+  for (int b = 0; b < P.nsys; b++)
+    for (float r : ms[b]->radii) {
+      int g = -1;
+      for (int t = 0; t < (int)P.radii.size(); t++)
+        if (fabsf(r - P.radii[t]) < FLT_MIN) { g = t; break; }
+      if (g < 0) { g = (int)P.radii.size(); P.radii.push_back(r); }
+      lmap[b].push_back(g);
+    }
+  P.ntypes = (int)P.radii.size();
+  if (P.ntypes > 16) {
+    errprint("agbnp3: %d distinct radii; at most 16 are supported.\n", P.ntypes);
+    return AGBNP_ERR;
+  }
+  std::vector<int> int2ext, asys, hrow, ws_stale0;
+  std::vector<float> R, q, galpha, igamma, sgamma, ialpha, salpha, hbcorr;
+  std::vector<double> vols;
+  std::vector<int2> ablk;
+  std::vector<WsDef> wsdef;
+  int a0 = 0, h0 = 0, ws0 = 0, blk0 = 0, tile0 = 0;
+  for (int b = 0; b < P.nsys; b++) {
+    const Model &m = *ms[b];
+    SysInfo S{};
+    S.a0 = a0; S.n = m.natoms; S.nh = m.nheavy; S.h0 = h0;
+    S.ws0 = ws0; S.nws = m.nws;
+    S.blk0 = blk0; S.nblk = (m.natoms + 255) / 256;
+    const int nb = (m.natoms + 255) / 256;
+    S.tile0 = tile0; S.ntile = nb * (nb + 1) / 2;
+    S.df64 = -0.5 * (1.0 / (double)m.eps_in - 1.0 / (double)m.eps_out);
+    S.df = (float)S.df64;
+    float rmax = 0.f;
+    for (int i = 0; i < m.nheavy; i++) rmax = std::max(rmax, m.R[i]);
+    S.rmax_heavy = rmax;
+    P.sinfo.push_back(S);
+    for (int i = 0; i < m.natoms; i++) {
+      int2ext.push_back(a0 + m.int2ext[i]);
+      asys.push_back(b);
+      P.rtype.push_back(lmap[b][m.rtype[i]]);
+      P.heavy.push_back(i < m.nheavy);
+      R.push_back(m.R[i]); q.push_back(m.q[i]); galpha.push_back(m.galpha[i]);
+      igamma.push_back(m.igamma[i]); sgamma.push_back(m.sgamma[i]);
+      ialpha.push_back(m.ialpha[i]); salpha.push_back(m.salpha[i]);
+      hbcorr.push_back(m.hbcorr[i]); vols.push_back(m.vols[i]);
+    }
+    for (int k = 0; k < m.nheavy; k++) hrow.push_back(a0 + k);
+    for (int k = 0; k < S.nblk; k++) ablk.push_back(make_int2(a0 + 256 * k, std::min(a0 + m.natoms, a0 + 256 * k + 256)));
+    for (WsDef w : m.wsdef) {
+      w.atom += a0;
+      for (int k = 0; k < 5; k++) w.p[k] += a0;
+      w.first += ws0;
+      wsdef.push_back(w);
+    }
+    for (int s : m.ws_stale0) ws_stale0.push_back(s >= 0 && s < m.natoms ? a0 + s : -1);
+    a0 += m.natoms; h0 += m.nheavy; ws0 += m.nws; blk0 += S.nblk; tile0 += S.ntile;
+  }
+  P.n = a0;
+  P.nh = h0;
+  Dev &d = P.d;
+  d = Dev{};
+  d.n = P.n;
+  d.nh = P.nh;
+  d.nsys = P.nsys;
+  d.ntypes = P.ntypes;
+  d.cap = P.cap;
+  d.options = options;
+  d.nws = ws0;
+  d.nwsdef = (int)wsdef.size();
+  d.nablk = blk0;
+  std::vector<float> ty((size_t)P.ntypes * P.ntypes * kTblN), ty2(ty.size());
+  for (int a = 0; a < P.ntypes; a++)
+    for (int b = 0; b < P.ntypes; b++) {
+      const float c = kRadInc;
+      const float ratio = (P.radii[a] - c) / P.radii[b];
+      const size_t t = (size_t)(a * P.ntypes + b) * kTblN;
+      build_table(ratio, &ty[t], &ty2[t], d.tdx, d.tdxinv);
+    }
+  std::vector<float2> tb(ty.size());
+  for (size_t k = 0; k < tb.size(); k++) tb[k] = make_float2(ty[k], ty2[k]);
+  auto &L = P.allocs;
+  if (upload(L, &d.sys, P.sinfo) || upload(L, &d.asys, asys) || upload(L, &d.hrow, hrow) ||
+      upload(L, &d.int2ext, int2ext) || upload(L, &d.R, R) || upload(L, &d.q, q) || upload(L, &d.rtype, P.rtype) ||
+      upload(L, &d.galpha, galpha) || upload(L, &d.igamma, igamma) || upload(L, &d.sgamma, sgamma) ||
+      upload(L, &d.ialpha, ialpha) || upload(L, &d.salpha, salpha) || upload(L, &d.hbcorr_f, hbcorr) ||
+      upload(L, &d.vols, vols) || upload(L, &d.tbl, tb) || upload(L, &d.ablk, ablk) ||
+      upload(L, &d.wsdef, wsdef) || upload(L, &d.ws_stale0, ws_stale0))
+    return AGBNP_ERR;
+  const size_t nn = std::max(P.n, 1), nhh = std::max(P.nh, 1), nws = std::max(ws0, 1);
+  if (dalloc(L, &d.pos, nn) || dalloc(L, &d.grid, (size_t)P.nsys) ||
+      dalloc(L, &d.cell_start, (size_t)P.nsys * (kMaxCells + 1)) || dalloc(L, &d.cell_atoms, nn) ||
+      dalloc(L, &d.cell_tmp, nn) || dalloc(L, &d.nne, nhh) || dalloc(L, &d.nbr, nhh * 256) ||
+      dalloc(L, &d.cand_nne, nhh) || dalloc(L, &d.cand_nbr, nhh * kCandCap) || dalloc(L, &d.ref_pos, nn) ||
+      dalloc(L, &d.nbl_state, 4) ||
+      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, Q_COUNT) || dalloc(L, &d.counters, 4) ||
+      dalloc(L, &d.acc_vol, nn) || dalloc(L, &d.acc_sa, nn) || dalloc(L, &d.acc_grad, nn * TG_N) ||
+      dalloc(L, &d.acc_hb, 3 * nn) || dalloc(L, &d.acc_derh, nn) || dalloc(L, &d.acc_beta, nn) ||
+      dalloc(L, &d.acc_gbA, nn) || dalloc(L, &d.acc_gbg, 3 * nn) || dalloc(L, &d.acc_ch, 8 * nn) ||
+      dalloc(L, &d.volumep_raw, nn) || dalloc(L, &d.sa_raw, nn) || dalloc(L, &d.sa_f, nn) ||
+      dalloc(L, &d.sp, nn) || dalloc(L, &d.spe, nn) || dalloc(L, &d.psvol, nn) || dalloc(L, &d.gammap, nn) ||
+      dalloc(L, &d.sp_f, nn) || dalloc(L, &d.spe_f, nn) || dalloc(L, &d.beta, nn) || dalloc(L, &d.br, nn) ||
+      dalloc(L, &d.brw, nn) || dalloc(L, &d.swf, nn) || dalloc(L, &d.br_f, nn) ||
+      dalloc(L, &d.gb_epart, (size_t)std::max(tile0, 1)) || dalloc(L, &d.A, nn) || dalloc(L, &d.q2ab, nn) ||
+      dalloc(L, &d.abrw, nn) || dalloc(L, &d.q2ab_f, nn) || dalloc(L, &d.abrw_f, nn) ||
+      dalloc(L, &d.deru, nn) || dalloc(L, &d.derv, nn) || dalloc(L, &d.derus, nn) || dalloc(L, &d.dervs, nn) ||
+      dalloc(L, &d.derh, nn) || dalloc(L, &d.deru_f, nn) || dalloc(L, &d.derv_f, nn) ||
+      dalloc(L, &d.derus_f, nn) || dalloc(L, &d.dervs_f, nn) || dalloc(L, &d.derh_f, nn) ||
+      dalloc(L, &d.gammap_f, nn) || dalloc(L, &d.ws, nws) || dalloc(L, &d.ws_part, nws) ||
+      dalloc(L, &d.epart, (size_t)EP_COUNT * std::max(blk0, 1)) || dalloc(L, &P.dx, nn) || dalloc(L, &P.dy, nn) ||
+      dalloc(L, &P.dz, nn) || dalloc(L, &P.dout, 15 * nn + 7 * (size_t)P.nsys))
+    return AGBNP_ERR;
+  HIPCHK(hipMemset(d.ws_part, 0, sizeof(double) * nws));
+  HIPCHK(hipMemset(d.nbl_state, 0, sizeof(int) * 4));
+  HIPCHK(hipHostMalloc((void **)&P.hstage, (18 * nn + 7 * (size_t)P.nsys) * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  HIPCHK(prepare_pairs(P.ntypes));
+  return build_items(P);
+}
 
-int run_pipeline(Instance *I, const double *x, const double *y, const double *z, double *sp, double *br, double *sa,
-                 double *dgbdr, double *dvwdr, double *decav, double *dehb, double *energies) {
-  Dev &d = I->d;
-  d.options = I->options;
-  hipStream_t s = I->stream;
-  const bool timed = (I->options & AGBNP3_OPT_TIMING) != 0;
+// ---------------------------------------------------------- the pipeline
+int run_pipeline(Pack &P, const double *x, const double *y, const double *z, double *sp, double *br, double *sa,
+                 double *dgbdr, double *dvwdr, double *decav, double *dehb, double *energies, bool capturing) {
+  Dev &d = P.d;
+  d.options = P.options;
+  d.cap = P.cap;
+  d.skin = P.skin;
+  d.reuse_every = P.reuse_every;
+  hipStream_t s = P.stream;
+  const bool timed = !capturing && (P.options & AGBNP3_OPT_TIMING) != 0;
   int k = 0;
   std::vector<hipEvent_t> *set = nullptr;
   if (timed) {
-    if ((size_t)I->ncalls_timed >= I->ev_sets.size()) I->ev_sets.emplace_back();
-    set = &I->ev_sets[I->ncalls_timed++];
+    if ((size_t)P.ncalls_timed >= P.ev_sets.size()) P.ev_sets.emplace_back();
+    set = &P.ev_sets[P.ncalls_timed++];
   }
   auto mark = [&](const char *name) {
     if (!timed) return;
@@ -437,11 +597,12 @@ int run_pipeline(Instance *I, const double *x, const double *y, const double *z,
       (void)hipEventCreate(&e);
       set->push_back(e);
     }
-    if ((size_t)k >= I->ev_names.size()) I->ev_names.push_back(name);
-    I->ev_names[k] = name;
+    if ((size_t)k >= P.ev_names.size()) P.ev_names.push_back(name);
+    P.ev_names[k] = name;
     (void)hipEventRecord((*set)[k], s);
     k++;
   };
+  const Tuning &t = P.tune;
   mark("start");
   launch_ingest(d, x, y, z, s);
   mark("ingest");
@@ -453,15 +614,15 @@ int run_pipeline(Instance *I, const double *x, const double *y, const double *z,
   mark("tree_volumes");
   launch_scale(d, s);
   mark("scale");
-  launch_descreen(d, s);
+  launch_descreen(d, t, s);
   mark("descreen");
   launch_born(d, s);
   mark("born");
-  launch_gbpair(d, s);
+  launch_gbpair(d, t, s);
   mark("gbpair");
   launch_q2ab(d, s);
   mark("q2ab");
-  launch_chain(d, s);
+  launch_chain(d, t, s);
   mark("chain");
   launch_ws_place(d, s);
   mark("ws_place");
@@ -477,16 +638,291 @@ int run_pipeline(Instance *I, const double *x, const double *y, const double *z,
   return AGBNP_OK;
 }
 
-int check_status(Instance *I, int *status_out) {
+// status word and largest neighbour-list row of the last call (waits)
+int read_status(Pack &P, int *status, int *maxrow) {
   int st = 0;
-  HIPCHK(hipMemcpyAsync(&st, I->d.status, sizeof(int), hipMemcpyDeviceToHost, I->stream));
-  HIPCHK(hipStreamSynchronize(I->stream));
-  *status_out = st;
+  unsigned long long c[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(&st, P.d.status, sizeof(int), hipMemcpyDeviceToHost, P.stream));
+  HIPCHK(hipMemcpyAsync(c, P.d.counters, sizeof(c), hipMemcpyDeviceToHost, P.stream));
+  HIPCHK(hipStreamSynchronize(P.stream));
+  *status = st;
+  if (maxrow) *maxrow = (int)c[3];
+  return AGBNP_OK;
+}
+
+int pack_sync(Pack &P, const char *who) {
+  int st = 0;
+  if (read_status(P, &st, nullptr) != AGBNP_OK) return AGBNP_ERR;
+  if (st) {
+    errprint("%s: device status 0x%x (capacity exceeded)\n", who, st);
+    return AGBNP_ERR;
+  }
+  return AGBNP_OK;
+}
+
+// neighbour-list row capacity for the next call: the smallest variant that
+// holds the longest row seen with a margin (smaller rows give the tree
+// kernels more waves per CU)
+void adapt_cap(Pack &P, int maxrow) {
+  const int want = maxrow + 8 <= 64 ? 64 : (maxrow + 8 <= 128 ? 128 : 256);
+  if (want != P.cap) {
+    P.cap = want;
+    drop_graph(P);
+  }
+}
+
+// Evaluate with host arrays (caller order, concatenated over the systems):
+// stage through pinned memory, retry once with wider rows when a row
+// overflowed, copy the outputs back.
+int pack_ener_host(Pack &P, const double *x, const double *y, const double *z, double *out /* 15n + 7 nsys */,
+                   const char *who) {
+  const size_t n = P.n;
+  double *h = P.hstage;
+  double *hin = h + 15 * n + 7 * (size_t)P.nsys;
+  memcpy(hin, x, n * sizeof(double));
+  memcpy(hin + n, y, n * sizeof(double));
+  memcpy(hin + 2 * n, z, n * sizeof(double));
+  hipStream_t s = P.stream;
+  HIPCHK(hipMemcpyAsync(P.dx, hin, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(P.dy, hin + n, n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(P.dz, hin + 2 * n, n * sizeof(double), hipMemcpyHostToDevice, s));
+  double *o = P.dout;
+  for (int attempt = 0; attempt < 3; attempt++) {
+    if (run_pipeline(P, P.dx, P.dy, P.dz, o, o + n, o + 2 * n, o + 3 * n, o + 6 * n, o + 9 * n, o + 12 * n,
+                     o + 15 * n, false) != AGBNP_OK)
+      return AGBNP_ERR;
+    int st = 0, maxrow = 0;
+    if (read_status(P, &st, &maxrow) != AGBNP_OK) return AGBNP_ERR;
+    if (st == 0) {
+      adapt_cap(P, maxrow);
+      break;
+    }
+    if ((st & ST_NBLIST_OVERFLOW) && P.cap < 256) {
+      P.cap = P.cap < 128 ? 128 : 256;     // wider rows; re-run the call
+      drop_graph(P);
+      continue;
+    }
+    errprint("%s: device capacity exceeded (status 0x%x).\n", who, st);
+    return AGBNP_ERR;
+  }
+  HIPCHK(hipMemcpyAsync(h, o, (15 * n + 7 * (size_t)P.nsys) * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (out != h) memcpy(out, h, (15 * n + 7 * (size_t)P.nsys) * sizeof(double));
+  return AGBNP_OK;
+}
+
+// ------------------------------------------------------------- timing
+int pack_kernel_times(Pack &P, float *ms, int nmax, char *names, int names_len) {
+  HIPCHK(hipStreamSynchronize(P.stream));
+  const int nk = P.ev_names.empty() ? 0 : (int)P.ev_names.size() - 1;
+  const int k = std::min(nmax, nk);
+  std::vector<double> acc(std::max(nk, 1), 0.0);
+  for (int c = 0; c < P.ncalls_timed; c++) {
+    const auto &set = P.ev_sets[c];
+    for (int i = 0; i < nk && i + 1 < (int)set.size(); i++) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, set[i], set[i + 1]);
+      acc[i] += t;
+    }
+  }
+  std::string all;
+  for (int i = 0; i < k; i++) {
+    if (ms) ms[i] = P.ncalls_timed ? (float)(acc[i] / P.ncalls_timed) : 0.f;
+    if (i) all += ";";
+    all += P.ev_names[i + 1];
+  }
+  if (names && names_len > 0) {
+    strncpy(names, all.c_str(), names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  P.ncalls_timed = 0;
+  return k;
+}
+
+// ------------------------------------------------------------ autotuner
+// Times the pipeline on the caller's coordinates for each candidate launch
+// shape and keeps, kernel by kernel, the fastest: rows per spline block,
+// column chunk of the descreening and chain items, persistent grid sizes.
+// Coordinate descent; every trial is one warm-up call and `reps` timed calls.
+enum : int { TUNE_ROWS = 0, TUNE_DESC_CHUNK, TUNE_CHAIN_CHUNK, TUNE_DESC_GRID, TUNE_GB_GRID, TUNE_CHAIN_GRID,
+             TUNE_CAP, TUNE_COUNT };
+
+void tuning_to_params(const Pack &P, int *p) {
+  p[TUNE_ROWS] = P.tune.rows;
+  p[TUNE_DESC_CHUNK] = P.tune.desc_chunk;
+  p[TUNE_CHAIN_CHUNK] = P.tune.chain_chunk;
+  p[TUNE_DESC_GRID] = P.tune.desc_grid;
+  p[TUNE_GB_GRID] = P.tune.gb_grid;
+  p[TUNE_CHAIN_GRID] = P.tune.chain_grid;
+  p[TUNE_CAP] = P.cap;
+}
+
+int params_to_tuning(Pack &P, const int *p) {
+  Tuning t;
+  t.rows = p[TUNE_ROWS];
+  t.desc_chunk = p[TUNE_DESC_CHUNK];
+  t.chain_chunk = p[TUNE_CHAIN_CHUNK];
+  t.desc_grid = p[TUNE_DESC_GRID];
+  t.gb_grid = p[TUNE_GB_GRID];
+  t.chain_grid = p[TUNE_CHAIN_GRID];
+  const int cap = p[TUNE_CAP];
+  if ((t.rows != 64 && t.rows != 128 && t.rows != 256) || t.desc_chunk < 0 || t.chain_chunk < 0 ||
+      t.desc_grid < 0 || t.gb_grid < 0 || t.chain_grid < 0 || (cap != 64 && cap != 128 && cap != 256) ||
+      (t.desc_chunk % 256) || (t.chain_chunk % 256)) {
+    errprint("agbnp3: invalid launch shape (rows %d, chunks %d/%d, grids %d/%d/%d, cap %d)\n", t.rows,
+             t.desc_chunk, t.chain_chunk, t.desc_grid, t.gb_grid, t.chain_grid, cap);
+    return AGBNP_ERR;
+  }
+  const bool rebuild = t.rows != P.tune.rows || t.desc_chunk != P.tune.desc_chunk ||
+                       t.chain_chunk != P.tune.chain_chunk;
+  P.tune = t;
+  P.cap = cap;
+  drop_graph(P);
+  return rebuild ? build_items(P) : AGBNP_OK;
+}
+
+int pack_tune(Pack &P, const double *x, const double *y, const double *z, int reps, char *report, int report_len) {
+  reps = std::max(1, reps);
+  const size_t n = P.n;
+  double *o = P.dout;
+  const int saved_opts = P.options;
+  std::string rep;
+  char line[256];
+  // one untimed call sizes the neighbour-list rows
+  {
+    P.options = saved_opts & ~AGBNP3_OPT_TIMING;
+    if (run_pipeline(P, x, y, z, o, o + n, o + 2 * n, o + 3 * n, o + 6 * n, o + 9 * n, o + 12 * n, o + 15 * n,
+                     false) != AGBNP_OK)
+      return AGBNP_ERR;
+    int st = 0, maxrow = 0;
+    if (read_status(P, &st, &maxrow) != AGBNP_OK) return AGBNP_ERR;
+    if (st & ST_NBLIST_OVERFLOW) {
+      P.cap = 256;
+    } else {
+      adapt_cap(P, maxrow);
+    }
+  }
+  // time one configuration: average ms per kernel name
+  auto trial = [&](std::map<std::string, double> &out) -> int {
+    P.options = (saved_opts & ~AGBNP3_OPT_TIMING);
+    if (run_pipeline(P, x, y, z, o, o + n, o + 2 * n, o + 3 * n, o + 6 * n, o + 9 * n, o + 12 * n, o + 15 * n,
+                     false) != AGBNP_OK)
+      return AGBNP_ERR;
+    P.options = saved_opts | AGBNP3_OPT_TIMING;
+    pack_kernel_times(P, nullptr, 0, nullptr, 0);
+    for (int r = 0; r < reps; r++)
+      if (run_pipeline(P, x, y, z, o, o + n, o + 2 * n, o + 3 * n, o + 6 * n, o + 9 * n, o + 12 * n, o + 15 * n,
+                       false) != AGBNP_OK)
+        return AGBNP_ERR;
+    int st = 0;
+    if (read_status(P, &st, nullptr) != AGBNP_OK || st) return AGBNP_ERR;
+    std::vector<float> ms(64);
+    std::vector<char> names(2048);
+    const int k = pack_kernel_times(P, ms.data(), 64, names.data(), (int)names.size());
+    out.clear();
+    std::string all(names.data());
+    size_t pos = 0;
+    for (int i = 0; i < k; i++) {
+      const size_t e = all.find(';', pos);
+      out[all.substr(pos, e == std::string::npos ? std::string::npos : e - pos)] = ms[i];
+      pos = e == std::string::npos ? all.size() : e + 1;
+    }
+    return AGBNP_OK;
+  };
+  int p[TUNE_COUNT];
+  tuning_to_params(P, p);
+  std::map<std::string, double> t;
+  int cu = 256;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cu = prop.multiProcessorCount;
+  }
+  // each step: a parameter, its candidates, the kernels whose time decides
+  struct Axis {
+    int param;
+    std::vector<int> cand;
+    std::vector<std::string> kernels;
+  };
+  const std::vector<Axis> axes = {
+      {TUNE_ROWS, {64, 128, 256}, {"descreen", "chain"}},
+      {TUNE_DESC_CHUNK, {0, 512, 1024, 2048, 4096}, {"descreen"}},
+      {TUNE_CHAIN_CHUNK, {0, 512, 1024, 2048, 4096}, {"chain"}},
+      {TUNE_DESC_GRID, {0, 2 * cu, 4 * cu, 8 * cu}, {"descreen"}},
+      {TUNE_GB_GRID, {0, 2 * cu, 4 * cu, 8 * cu}, {"gbpair"}},
+      {TUNE_CHAIN_GRID, {0, 2 * cu, 4 * cu, 8 * cu}, {"chain"}},
+  };
+  for (const Axis &ax : axes) {
+    int best = p[ax.param];
+    double best_t = 1e30;
+    for (int c : ax.cand) {
+      int q[TUNE_COUNT];
+      memcpy(q, p, sizeof(q));
+      q[ax.param] = c;
+      if (params_to_tuning(P, q) != AGBNP_OK || trial(t) != AGBNP_OK) {
+        P.options = saved_opts;
+        return AGBNP_ERR;
+      }
+      double tt = 0.0;
+      for (const auto &kname : ax.kernels) tt += t[kname];
+      snprintf(line, sizeof(line), "param %d = %d: %.4f ms\n", ax.param, c, tt);
+      rep += line;
+      if (tt < best_t) { best_t = tt; best = c; }
+    }
+    p[ax.param] = best;
+  }
+  P.options = saved_opts;
+  if (params_to_tuning(P, p) != AGBNP_OK) return AGBNP_ERR;
+  snprintf(line, sizeof(line), "chosen: rows %d desc_chunk %d chain_chunk %d desc_grid %d gb_grid %d chain_grid %d cap %d\n",
+           p[0], p[1], p[2], p[3], p[4], p[5], p[6]);
+  rep += line;
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return AGBNP_OK;
+}
+
+// caller arrays of float_i as double: the caller's memory itself when
+// float_i is double, else a converted copy in `tmp`
+template <typename T>
+const double *as_double(const T *p, size_t n, std::vector<double> &tmp) {
+  if constexpr (std::is_same<T, double>::value) {
+    return p;
+  } else {
+    tmp.assign(p, p + n);
+    return tmp.data();
+  }
+}
+template <typename T>
+void store(T *dst, const double *src, size_t n) {
+  if constexpr (std::is_same<T, double>::value) {
+    memcpy(dst, src, n * sizeof(double));
+  } else {
+    for (size_t k = 0; k < n; k++) dst[k] = (T)src[k];
+  }
+}
+
+int pack_capture(Pack &P, const double *x, const double *y, const double *z, double *sp, double *br, double *sa,
+                 double *dgbdr, double *dvwdr, double *decav, double *dehb, double *energies) {
+  drop_graph(P);
+  HIPCHK(hipStreamSynchronize(P.stream));
+  HIPCHK(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+  const int rc = run_pipeline(P, x, y, z, sp, br, sa, dgbdr, dvwdr, decav, dehb, energies, true);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(P.stream, &g);
+  if (rc != AGBNP_OK || e != hipSuccess) {
+    errprint("agbnp3: graph capture failed: %s\n", hipGetErrorString(e));
+    if (g) (void)hipGraphDestroy(g);
+    return AGBNP_ERR;
+  }
+  P.graph = g;
+  HIPCHK(hipGraphInstantiate(&P.gexec, P.graph, nullptr, nullptr, 0));
   return AGBNP_OK;
 }
 
 }  // namespace
-
 // ======================================================================== API
 extern "C" {
 
@@ -509,10 +945,21 @@ void agbnp3_terminate(void) {
     return;
   }
   for (auto &p : g_slots) {
-    free_instance(p);
+    if (p) {
+      free_pack(p->p);
+      delete p;
+    }
     p = nullptr;
   }
   g_slots.clear();
+  for (auto &b : g_batches) {
+    if (b) {
+      free_pack(b->p);
+      delete b;
+    }
+    b = nullptr;
+  }
+  g_batches.clear();
   g_initialized = false;
 }
 
@@ -534,7 +981,8 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
     errprint("agbnp3_new(): invalid arguments.\n");
     return AGBNP_ERR;
   }
-  Instance *I = new Instance();
+  Instance *inst = new Instance();
+  Model *I = &inst->m;
   const int n = natoms;
   I->natoms = n;
   I->verbose = verbose;
@@ -543,7 +991,7 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
   for (int k = 0; k < nhydrogen; k++) {
     if (ihydrogen[k] < 0 || ihydrogen[k] >= n) {
       errprint("agbnp3_new(): hydrogen index %d out of range.\n", ihydrogen[k]);
-      delete I;
+      delete inst;
       return AGBNP_ERR;
     }
     ish[ihydrogen[k]] = 1;
@@ -569,7 +1017,7 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
     I->R[i] = f((double)r[e] + (double)kRadInc);     // radius + 0.5 in double, stored FP32 (agbnp3.c:210)
     if (!(I->R[i] > kRadInc)) {
       errprint("agbnp3_new(): atom %d has a non-positive radius.\n", e);
-      delete I;
+      delete inst;
       return AGBNP_ERR;
     }
     I->q[i] = f(charge[e]);
@@ -588,7 +1036,7 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
   // connection table, remapped (agbnp3.c:296-317)
   if (conntbl->natoms < n) {
     errprint("agbnp3_new(): connection table has %d rows for %d atoms.\n", conntbl->natoms, n);
-    delete I;
+    delete inst;
     return AGBNP_ERR;
   }
   I->cnne.resize(n);
@@ -604,7 +1052,7 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
       const int nbr = conntbl->neighl[e][k];
       if (nbr < 0 || nbr >= n) {
         errprint("agbnp3_new(): connection table entry %d of atom %d out of range.\n", nbr, e);
-        delete I;
+        delete inst;
         return AGBNP_ERR;
       }
       I->cnl[I->coff[i] + k] = I->ext2int[nbr];
@@ -625,7 +1073,7 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
   }
   if (I->ntypes > 16) {
     errprint("agbnp3_new(): %d distinct radii; at most 16 are supported.\n", I->ntypes);
-    delete I;
+    delete inst;
     return AGBNP_ERR;
   }
   const int nt = I->ntypes;
@@ -639,15 +1087,16 @@ int agbnp3_new(int *tag, int natoms, float_i *x, float_i *y, float_i *z, float_i
       build_table(ratio, &I->ty[t], &I->ty2[t], I->tdx, I->tdxinv);
     }
   build_water_sites(I);
-  if (setup_device(I) != AGBNP_OK) {
-    free_instance(I);
+  if (build_pack(inst->p, {I}, 0) != AGBNP_OK) {
+    free_pack(inst->p);
+    delete inst;
     return AGBNP_ERR;
   }
   // lowest free slot (agbnp3.c:144-170)
   int slot = 0;
   while (slot < (int)g_slots.size() && g_slots[slot]) slot++;
   if (slot == (int)g_slots.size()) g_slots.push_back(nullptr);
-  g_slots[slot] = I;
+  g_slots[slot] = inst;
   *tag = slot;
   return AGBNP_OK;
 }
@@ -656,7 +1105,8 @@ int agbnp3_delete(int tag) {
   std::lock_guard<std::mutex> lk(g_mutex);
   Instance *I = get_instance(tag, "agbnp3_delete()");
   if (!I) return AGBNP_ERR;
-  free_instance(I);
+  free_pack(I->p);
+  delete I;
   g_slots[tag] = nullptr;
   return AGBNP_OK;
 }
@@ -668,7 +1118,8 @@ int agbnp_delete(int tag) { return agbnp3_delete(tag); }
 int agbnp3_set_options(int tag, int options) {
   Instance *I = get_instance(tag, "agbnp3_set_options()");
   if (!I) return AGBNP_ERR;
-  I->options = options;
+  if ((I->p.options ^ options) & ~AGBNP3_OPT_TIMING) drop_graph(I->p);
+  I->p.options = options;
   return AGBNP_OK;
 }
 
@@ -677,30 +1128,24 @@ int agbnp3_ener_device(int tag, const double *x, const double *y, const double *
                        double *energies) {
   Instance *I = get_instance(tag, "agbnp3_ener_device()");
   if (!I) return AGBNP_ERR;
-  if (I->ws_error_atom >= 0) {
+  if (I->m.ws_error_atom >= 0) {
     errprint("agbnp3_create_ws_atoms(): error in agbnp3_create_ws_atoms_trigonal_oop(): atom %d needs 3 bonds.\n",
-             I->int2ext[I->ws_error_atom]);
+             I->m.int2ext[I->m.ws_error_atom]);
     errprint("agbnp3_ener(): error in agbnp3_total_energy().\n");
     return AGBNP_ERR;
   }
-  return run_pipeline(I, x, y, z, sp, br, surf_area, dgbdr, dvwdr, decav, dehb, energies);
+  return run_pipeline(I->p, x, y, z, sp, br, surf_area, dgbdr, dvwdr, decav, dehb, energies, false);
 }
 
 int agbnp3_sync(int tag) {
   Instance *I = get_instance(tag, "agbnp3_sync()");
   if (!I) return AGBNP_ERR;
-  int st = 0;
-  if (check_status(I, &st) != AGBNP_OK) return AGBNP_ERR;
-  if (st) {
-    errprint("agbnp3: device status 0x%x (capacity exceeded)\n", st);
-    return AGBNP_ERR;
-  }
-  return AGBNP_OK;
+  return pack_sync(I->p, "agbnp3_sync()");
 }
 
 void *agbnp3_stream(int tag) {
   Instance *I = get_instance(tag, "agbnp3_stream()");
-  return I ? (void *)I->stream : nullptr;
+  return I ? (void *)I->p.stream : nullptr;
 }
 
 int agbnp3_ener(int tag, int init, float_i *x, float_i *y, float_i *z, float_i *sp, float_i *br,
@@ -710,42 +1155,29 @@ int agbnp3_ener(int tag, int init, float_i *x, float_i *y, float_i *z, float_i *
   (void)init;
   Instance *I = get_instance(tag, "agbnp3_agb_energy()");
   if (!I) return AGBNP_ERR;
-  const size_t n = I->natoms;
-  double *h = I->hstage;
-  double *hin = h + 15 * n + 7;
-  memcpy(hin, x, n * sizeof(double));
-  memcpy(hin + n, y, n * sizeof(double));
-  memcpy(hin + 2 * n, z, n * sizeof(double));
-  hipStream_t s = I->stream;
-  HIPCHK(hipMemcpyAsync(I->dx, hin, n * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(I->dy, hin + n, n * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(I->dz, hin + 2 * n, n * sizeof(double), hipMemcpyHostToDevice, s));
-  double *o = I->dout;
-  for (int attempt = 0; attempt < 2; attempt++) {
-    if (agbnp3_ener_device(tag, I->dx, I->dy, I->dz, o, o + n, o + 2 * n, o + 3 * n, o + 6 * n, o + 9 * n,
-                           o + 12 * n, o + 15 * n) != AGBNP_OK) {
-      errprint("agbnp3_ener(): error in agbnp3_total_energy().\n");
-      return AGBNP_ERR;
-    }
-    int st = 0;
-    if (check_status(I, &st) != AGBNP_OK) return AGBNP_ERR;
-    if (st == 0) break;
-    if ((st & ST_NBLIST_OVERFLOW) && I->d.cap < 256 && attempt == 0) {
-      I->d.cap = 256;   // wider rows, larger LDS variants; re-run the call
-      continue;
-    }
-    errprint("agbnp3_ener(): device capacity exceeded (status 0x%x).\n", st);
+  if (I->m.ws_error_atom >= 0) {
+    errprint("agbnp3_create_ws_atoms(): error in agbnp3_create_ws_atoms_trigonal_oop(): atom %d needs 3 bonds.\n",
+             I->m.int2ext[I->m.ws_error_atom]);
+    errprint("agbnp3_ener(): error in agbnp3_total_energy().\n");
     return AGBNP_ERR;
   }
-  HIPCHK(hipMemcpyAsync(h, o, (15 * n + 7) * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  memcpy(sp, h, n * sizeof(double));
-  memcpy(br, h + n, n * sizeof(double));
-  memcpy(surf_area, h + 2 * n, n * sizeof(double));
-  memcpy(dgbdr, h + 3 * n, 3 * n * sizeof(double));
-  memcpy(dvwdr, h + 6 * n, 3 * n * sizeof(double));
-  memcpy(decav, h + 9 * n, 3 * n * sizeof(double));
-  memcpy(dehb, h + 12 * n, 3 * n * sizeof(double));
+  const size_t n = I->m.natoms;
+  // float_i is double, or float in the SINGLE_PREC_INTERFACE build of the
+  // library (agbnp3.h); the engine's boundary arrays are double either way
+  std::vector<double> xin, yin, zin;
+  const double *xd = as_double(x, n, xin), *yd = as_double(y, n, yin), *zd = as_double(z, n, zin);
+  double *h = I->p.hstage;    // outputs land at the start of the staging area
+  if (pack_ener_host(I->p, xd, yd, zd, h, "agbnp3_ener()") != AGBNP_OK) {
+    errprint("agbnp3_ener(): error in agbnp3_total_energy().\n");
+    return AGBNP_ERR;
+  }
+  store(sp, h, n);
+  store(br, h + n, n);
+  store(surf_area, h + 2 * n, n);
+  store(&dgbdr[0][0], h + 3 * n, 3 * n);
+  store(&dvwdr[0][0], h + 6 * n, 3 * n);
+  store(&decav[0][0], h + 9 * n, 3 * n);
+  store(&dehb[0][0], h + 12 * n, 3 * n);
   const double *e = h + 15 * n;
   *egb = e[0];
   *evdw = e[1];
@@ -778,26 +1210,26 @@ int agbnp3_q4_tables(int ntypes, const float *radii, float *y, float *y2, float 
 int agbnp3_natoms(int tag, int *natoms, int *nheavy) {
   Instance *I = get_instance(tag, "agbnp3_natoms()");
   if (!I) return AGBNP_ERR;
-  if (natoms) *natoms = I->natoms;
-  if (nheavy) *nheavy = I->nheavy;
+  if (natoms) *natoms = I->m.natoms;
+  if (nheavy) *nheavy = I->m.nheavy;
   return AGBNP_OK;
 }
 
 int agbnp3_get_int2ext(int tag, int *int2ext) {
   Instance *I = get_instance(tag, "agbnp3_get_int2ext()");
   if (!I) return AGBNP_ERR;
-  memcpy(int2ext, I->int2ext.data(), sizeof(int) * I->natoms);
+  memcpy(int2ext, I->m.int2ext.data(), sizeof(int) * I->m.natoms);
   return AGBNP_OK;
 }
 
 int agbnp3_get_near_list(int tag, int *nne, int *nbr, int cap) {
   Instance *I = get_instance(tag, "agbnp3_get_near_list()");
   if (!I) return AGBNP_ERR;
-  const int nh = I->nheavy, dc = I->d.cap;
+  const int nh = I->m.nheavy, dc = I->p.d.cap;
   std::vector<int> hn(std::max(nh, 1)), hb((size_t)std::max(nh, 1) * dc);
-  HIPCHK(hipStreamSynchronize(I->stream));
-  HIPCHK(hipMemcpy(hn.data(), I->d.nne, sizeof(int) * nh, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(hb.data(), I->d.nbr, sizeof(int) * (size_t)nh * dc, hipMemcpyDeviceToHost));
+  HIPCHK(hipStreamSynchronize(I->p.stream));
+  HIPCHK(hipMemcpy(hn.data(), I->p.d.nne, sizeof(int) * nh, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(hb.data(), I->p.d.nbr, sizeof(int) * (size_t)nh * dc, hipMemcpyDeviceToHost));
   for (int i = 0; i < nh; i++) {
     if (nne) nne[i] = hn[i];
     if (nbr)
@@ -809,16 +1241,16 @@ int agbnp3_get_near_list(int tag, int *nne, int *nbr, int cap) {
 int agbnp3_get_tables(int tag, float *y, float *y2, float *radii) {
   Instance *I = get_instance(tag, "agbnp3_get_tables()");
   if (!I) return AGBNP_ERR;
-  if (y) memcpy(y, I->ty.data(), sizeof(float) * I->ty.size());
-  if (y2) memcpy(y2, I->ty2.data(), sizeof(float) * I->ty2.size());
-  if (radii) memcpy(radii, I->radii.data(), sizeof(float) * I->ntypes);
-  return I->ntypes;
+  if (y) memcpy(y, I->m.ty.data(), sizeof(float) * I->m.ty.size());
+  if (y2) memcpy(y2, I->m.ty2.data(), sizeof(float) * I->m.ty2.size());
+  if (radii) memcpy(radii, I->m.radii.data(), sizeof(float) * I->m.ntypes);
+  return I->m.ntypes;
 }
 
 int agbnp3_get_internal(int tag, const char *name, double *out) {
   Instance *I = get_instance(tag, "agbnp3_get_internal()");
   if (!I) return AGBNP_ERR;
-  const Dev &d = I->d;
+  const Dev &d = I->p.d;
   const double *src = nullptr;
   struct { const char *n; const double *p; } tab[] = {
       {"volumep_raw", d.volumep_raw}, {"sa_raw", d.sa_raw}, {"sa_f", d.sa_f}, {"sp", d.sp}, {"spe", d.spe},
@@ -831,8 +1263,8 @@ int agbnp3_get_internal(int tag, const char *name, double *out) {
     errprint("agbnp3_get_internal(): unknown array '%s'\n", name);
     return AGBNP_ERR;
   }
-  HIPCHK(hipStreamSynchronize(I->stream));
-  HIPCHK(hipMemcpy(out, src, sizeof(double) * I->natoms, hipMemcpyDeviceToHost));
+  HIPCHK(hipStreamSynchronize(I->p.stream));
+  HIPCHK(hipMemcpy(out, src, sizeof(double) * I->m.natoms, hipMemcpyDeviceToHost));
   return AGBNP_OK;
 }
 
@@ -840,11 +1272,11 @@ int agbnp3_get_counts(int tag, long long *counts) {
   Instance *I = get_instance(tag, "agbnp3_get_counts()");
   if (!I) return AGBNP_ERR;
   unsigned long long c[4];
-  HIPCHK(hipStreamSynchronize(I->stream));
-  HIPCHK(hipMemcpy(c, I->d.counters, sizeof(c), hipMemcpyDeviceToHost));
+  HIPCHK(hipStreamSynchronize(I->p.stream));
+  HIPCHK(hipMemcpy(c, I->p.d.counters, sizeof(c), hipMemcpyDeviceToHost));
   counts[0] = (long long)c[0];
   counts[1] = (long long)c[1];
-  counts[2] = I->nws;
+  counts[2] = I->m.nws;
   counts[3] = (long long)c[2];
   counts[4] = (long long)c[3];
   return AGBNP_OK;
@@ -853,30 +1285,215 @@ int agbnp3_get_counts(int tag, long long *counts) {
 int agbnp3_kernel_times(int tag, float *ms, int nmax, char *names, int names_len) {
   Instance *I = get_instance(tag, "agbnp3_kernel_times()");
   if (!I) return AGBNP_ERR;
-  HIPCHK(hipStreamSynchronize(I->stream));
-  const int nk = I->ev_names.empty() ? 0 : (int)I->ev_names.size() - 1;
-  const int k = std::min(nmax, nk);
-  std::vector<double> acc(std::max(nk, 1), 0.0);
-  for (int c = 0; c < I->ncalls_timed; c++) {
-    const auto &set = I->ev_sets[c];
-    for (int i = 0; i < nk && i + 1 < (int)set.size(); i++) {
-      float t = 0.f;
-      (void)hipEventElapsedTime(&t, set[i], set[i + 1]);
-      acc[i] += t;
+  return pack_kernel_times(I->p, ms, nmax, names, names_len);
+}
+
+int agbnp3_get_tuning(int tag, int *params, int nparams) {
+  Instance *I = get_instance(tag, "agbnp3_get_tuning()");
+  if (!I || nparams < TUNE_COUNT) return AGBNP_ERR;
+  tuning_to_params(I->p, params);
+  return AGBNP_OK;
+}
+
+int agbnp3_set_tuning(int tag, const int *params, int nparams) {
+  Instance *I = get_instance(tag, "agbnp3_set_tuning()");
+  if (!I || nparams < TUNE_COUNT) return AGBNP_ERR;
+  return params_to_tuning(I->p, params);
+}
+
+int agbnp3_tune(int tag, const double *x, const double *y, const double *z, int reps, char *report, int report_len) {
+  Instance *I = get_instance(tag, "agbnp3_tune()");
+  if (!I) return AGBNP_ERR;
+  return pack_tune(I->p, x, y, z, reps, report, report_len);
+}
+
+// ------------------------------------------------------------------ batches
+int agbnp3_batch_new(int *batch, int ntags, const int *tags) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!g_initialized) {
+    errprint("agbnp3_batch_new(): agbnp library is not initialized.\n");
+    return AGBNP_ERR;
+  }
+  if (!batch || ntags <= 0 || !tags) {
+    errprint("agbnp3_batch_new(): invalid arguments.\n");
+    return AGBNP_ERR;
+  }
+  std::vector<const Model *> ms;
+  Batch *B = new Batch();
+  int options = -1;
+  for (int k = 0; k < ntags; k++) {
+    Instance *I = get_instance(tags[k], "agbnp3_batch_new()");
+    if (!I) {
+      delete B;
+      return AGBNP_ERR;
     }
+    const int o = I->p.options & ~AGBNP3_OPT_TIMING;
+    if (options >= 0 && o != options) {
+      errprint("agbnp3_batch_new(): instances %d and %d have different options.\n", tags[0], tags[k]);
+      delete B;
+      return AGBNP_ERR;
+    }
+    options = o;
+    ms.push_back(&I->m);
+    B->tags.push_back(tags[k]);
+    B->ws_error.push_back(I->m.ws_error_atom >= 0 ? I->m.int2ext[I->m.ws_error_atom] : -1);
   }
-  std::string all;
-  for (int i = 0; i < k; i++) {
-    ms[i] = I->ncalls_timed ? (float)(acc[i] / I->ncalls_timed) : 0.f;
-    if (i) all += ";";
-    all += I->ev_names[i + 1];
+  if (build_pack(B->p, ms, options) != AGBNP_OK) {
+    free_pack(B->p);
+    delete B;
+    return AGBNP_ERR;
   }
-  if (names && names_len > 0) {
-    strncpy(names, all.c_str(), names_len - 1);
-    names[names_len - 1] = 0;
+  int slot = 0;
+  while (slot < (int)g_batches.size() && g_batches[slot]) slot++;
+  if (slot == (int)g_batches.size()) g_batches.push_back(nullptr);
+  g_batches[slot] = B;
+  *batch = slot;
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_delete(int batch) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  Batch *B = get_batch(batch, "agbnp3_batch_delete()");
+  if (!B) return AGBNP_ERR;
+  free_pack(B->p);
+  delete B;
+  g_batches[batch] = nullptr;
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_offsets(int batch, int *nsys, int *offsets) {
+  Batch *B = get_batch(batch, "agbnp3_batch_offsets()");
+  if (!B) return AGBNP_ERR;
+  if (nsys) *nsys = B->p.nsys;
+  if (offsets) {
+    for (int b = 0; b < B->p.nsys; b++) offsets[b] = B->p.sinfo[b].a0;
+    offsets[B->p.nsys] = B->p.n;
   }
-  I->ncalls_timed = 0;
-  return k;
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_set_options(int batch, int options) {
+  Batch *B = get_batch(batch, "agbnp3_batch_set_options()");
+  if (!B) return AGBNP_ERR;
+  if ((B->p.options ^ options) & ~AGBNP3_OPT_TIMING) drop_graph(B->p);
+  B->p.options = options;
+  return AGBNP_OK;
+}
+
+static int set_reuse(Pack &P, float skin, int every, const char *who) {
+  if (!(skin >= 0.f) || skin > 4.f || every < 0) {
+    errprint("%s: invalid skin %g or rebuild period %d.\n", who, skin, every);
+    return AGBNP_ERR;
+  }
+  HIPCHK(hipStreamSynchronize(P.stream));
+  HIPCHK(hipMemset(P.d.nbl_state, 0, sizeof(int) * 4));
+  P.skin = skin;
+  P.reuse_every = every;
+  drop_graph(P);
+  return AGBNP_OK;
+}
+
+int agbnp3_set_nblist_reuse(int tag, float skin, int every) {
+  Instance *I = get_instance(tag, "agbnp3_set_nblist_reuse()");
+  if (!I) return AGBNP_ERR;
+  return set_reuse(I->p, skin, every, "agbnp3_set_nblist_reuse()");
+}
+
+int agbnp3_nblist_rebuilds(int tag, int *calls, int *rebuilds) {
+  Instance *I = get_instance(tag, "agbnp3_nblist_rebuilds()");
+  if (!I) return AGBNP_ERR;
+  int st[4];
+  HIPCHK(hipStreamSynchronize(I->p.stream));
+  HIPCHK(hipMemcpy(st, I->p.d.nbl_state, sizeof(st), hipMemcpyDeviceToHost));
+  if (calls) *calls = st[1];
+  if (rebuilds) *rebuilds = st[3];
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_set_nblist_reuse(int batch, float skin, int every) {
+  Batch *B = get_batch(batch, "agbnp3_batch_set_nblist_reuse()");
+  if (!B) return AGBNP_ERR;
+  return set_reuse(B->p, skin, every, "agbnp3_batch_set_nblist_reuse()");
+}
+
+static int batch_topology_ok(Batch *B, const char *who) {
+  for (size_t k = 0; k < B->ws_error.size(); k++)
+    if (B->ws_error[k] >= 0) {
+      errprint("%s: instance %d: water-site topology error at atom %d (TRIGONAL_OOP needs 3 bonds).\n", who,
+               B->tags[k], B->ws_error[k]);
+      return 0;
+    }
+  return 1;
+}
+
+int agbnp3_batch_ener_device(int batch, const double *x, const double *y, const double *z, double *sp, double *br,
+                             double *surf_area, double *dgbdr, double *dvwdr, double *decav, double *dehb,
+                             double *energies) {
+  Batch *B = get_batch(batch, "agbnp3_batch_ener_device()");
+  if (!B || !batch_topology_ok(B, "agbnp3_batch_ener_device()")) return AGBNP_ERR;
+  return run_pipeline(B->p, x, y, z, sp, br, surf_area, dgbdr, dvwdr, decav, dehb, energies, false);
+}
+
+int agbnp3_batch_ener(int batch, const double *x, const double *y, const double *z, double *out) {
+  Batch *B = get_batch(batch, "agbnp3_batch_ener()");
+  if (!B || !batch_topology_ok(B, "agbnp3_batch_ener()")) return AGBNP_ERR;
+  return pack_ener_host(B->p, x, y, z, out, "agbnp3_batch_ener()");
+}
+
+int agbnp3_batch_sync(int batch) {
+  Batch *B = get_batch(batch, "agbnp3_batch_sync()");
+  if (!B) return AGBNP_ERR;
+  return pack_sync(B->p, "agbnp3_batch_sync()");
+}
+
+void *agbnp3_batch_stream(int batch) {
+  Batch *B = get_batch(batch, "agbnp3_batch_stream()");
+  return B ? (void *)B->p.stream : nullptr;
+}
+
+int agbnp3_batch_graph_capture(int batch, const double *x, const double *y, const double *z, double *sp, double *br,
+                               double *surf_area, double *dgbdr, double *dvwdr, double *decav, double *dehb,
+                               double *energies) {
+  Batch *B = get_batch(batch, "agbnp3_batch_graph_capture()");
+  if (!B || !batch_topology_ok(B, "agbnp3_batch_graph_capture()")) return AGBNP_ERR;
+  return pack_capture(B->p, x, y, z, sp, br, surf_area, dgbdr, dvwdr, decav, dehb, energies);
+}
+
+int agbnp3_batch_graph_launch(int batch) {
+  Batch *B = get_batch(batch, "agbnp3_batch_graph_launch()");
+  if (!B) return AGBNP_ERR;
+  if (!B->p.gexec) {
+    errprint("agbnp3_batch_graph_launch(): no captured graph (capture again after a change of shape or options).\n");
+    return AGBNP_ERR;
+  }
+  HIPCHK(hipGraphLaunch(B->p.gexec, B->p.stream));
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_kernel_times(int batch, float *ms, int nmax, char *names, int names_len) {
+  Batch *B = get_batch(batch, "agbnp3_batch_kernel_times()");
+  if (!B) return AGBNP_ERR;
+  return pack_kernel_times(B->p, ms, nmax, names, names_len);
+}
+
+int agbnp3_batch_get_tuning(int batch, int *params, int nparams) {
+  Batch *B = get_batch(batch, "agbnp3_batch_get_tuning()");
+  if (!B || nparams < TUNE_COUNT) return AGBNP_ERR;
+  tuning_to_params(B->p, params);
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_set_tuning(int batch, const int *params, int nparams) {
+  Batch *B = get_batch(batch, "agbnp3_batch_set_tuning()");
+  if (!B || nparams < TUNE_COUNT) return AGBNP_ERR;
+  return params_to_tuning(B->p, params);
+}
+
+int agbnp3_batch_tune(int batch, const double *x, const double *y, const double *z, int reps, char *report,
+                      int report_len) {
+  Batch *B = get_batch(batch, "agbnp3_batch_tune()");
+  if (!B || !batch_topology_ok(B, "agbnp3_batch_tune()")) return AGBNP_ERR;
+  return pack_tune(B->p, x, y, z, reps, report, report_len);
 }
 
 }  // extern "C"

@@ -8,7 +8,8 @@ namespace agb {
 
 constexpr int kEpi = 256;
 
-// block sum of up to 4 doubles into epart[slot][blockIdx.x]
+// block sum of up to 4 doubles into epart[slot][blockIdx.x]; an atom block
+// never spans two systems
 __device__ void block_partials(const Dev &d, const int *slots, const double *v, int nv) {
   __shared__ double red[4][kEpi / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -20,7 +21,7 @@ __device__ void block_partials(const Dev &d, const int *slots, const double *v, 
   if (threadIdx.x < nv) {
     double s = 0.0;
     for (int q = 0; q < kEpi / 64; q++) s += red[threadIdx.x][q];
-    d.epart[(size_t)slots[threadIdx.x] * d.epart_stride + blockIdx.x] = s;
+    d.epart[(size_t)slots[threadIdx.x] * d.nablk + blockIdx.x] = s;
   }
 }
 
@@ -29,10 +30,12 @@ __device__ void block_partials(const Dev &d, const int *slots, const double *v, 
 // (agbnp3.c:1104-1117); E_cav, E_corr_cav, molecular volume
 // (agbnp3.c:1390-1411)
 __global__ __launch_bounds__(kEpi) void k_scale(Dev d) {
-  const int i = blockIdx.x * kEpi + threadIdx.x;
+  const int2 blk = d.ablk[blockIdx.x];
+  const int i = blk.x + threadIdx.x;
   double ecav = 0.0, ecorr = 0.0, mv = 0.0;
-  if (i < d.n) {
-    if (i < d.nh) {
+  if (i < blk.y) {
+    const SysInfo &S = d.sys[d.asys[i]];
+    if (i < S.a0 + S.nh) {
       const double R = d.R[i];
       const double vraw = d.vols[i] + fix_get(d.acc_vol[i]);
       const double a = 4.0 * kPi * R * R + fix_get(d.acc_sa[i]);
@@ -71,19 +74,18 @@ __global__ __launch_bounds__(kEpi) void k_scale(Dev d) {
 }
 
 void launch_scale(const Dev &d, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale, dim3(scale_blocks(d)), dim3(kEpi), 0, s, d);
+  if (d.nablk) hipLaunchKernelGGL(k_scale, dim3(d.nablk), dim3(kEpi), 0, s, d);
 }
-int scale_blocks(const Dev &d) { return (d.n + kEpi - 1) / kEpi; }
 
 // ------------------------------------------------------------ Born radii
 // beta filter agbnp3_swf_invbr (agbnp3.c:1174-1188), agbnp3_born_radii
 // (agbnp3_utils.c:615-670), E_vdW (agbnp3.c:1466-1487), GB self energy
 __global__ __launch_bounds__(kEpi) void k_born(Dev d) {
-  const int i = blockIdx.x * kEpi + threadIdx.x;
+  const int2 blk = d.ablk[blockIdx.x];
+  const int i = blk.x + threadIdx.x;
   double evdw = 0.0, ecorr = 0.0, eself = 0.0;
-  if (i < d.n) {
-    double s = 0.0;
-    for (int c = 0; c < d.nchunk_desc; c++) s += d.beta_part[(size_t)c * d.n + i];
+  if (i < blk.y) {
+    const double s = fix_get(d.acc_beta[i]);
     const double beta = 1.0 / ((double)d.R[i] - kRadInc) - s / (4.0 * kPi);
     double bf, sw;
     if (beta < 0.0) {
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(kEpi) void k_born(Dev d) {
     evdw = (double)d.ialpha[i] * a3;
     ecorr = (double)d.salpha[i] * a3;
     const double q = d.q[i];
-    eself = d.df * q * q / B;
+    eself = d.sys[d.asys[i]].df64 * q * q / B;
   }
   const int slots[3] = {EP_VDW, EP_CORR_VDW, EP_SELF};
   const double v[3] = {evdw, ecorr, eself};
@@ -111,9 +113,8 @@ __global__ __launch_bounds__(kEpi) void k_born(Dev d) {
 }
 
 void launch_born(const Dev &d, hipStream_t s) {
-  hipLaunchKernelGGL(k_born, dim3(born_blocks(d)), dim3(kEpi), 0, s, d);
+  if (d.nablk) hipLaunchKernelGGL(k_born, dim3(d.nablk), dim3(kEpi), 0, s, d);
 }
-int born_blocks(const Dev &d) { return (d.n + kEpi - 1) / kEpi; }
 
 // -------------------------------------------------- chain-rule factors
 // q2ab = (q^2 + A B) swf', abrw = alpha brw swf' (kernels.c:433-450)
@@ -140,22 +141,21 @@ void launch_q2ab(const Dev &d, hipStream_t s) {
 __global__ __launch_bounds__(kEpi) void k_uv(Dev d) {
   const int i = blockIdx.x * kEpi + threadIdx.x;
   if (i >= d.n) return;
+  const SysInfo &S = d.sys[d.asys[i]];
+  const bool heavy = i < S.a0 + S.nh;
   double U = 0.0, V = 0.0;
-  if (i < d.nh) {
-    for (int c = 0; c < d.nchunk_ch; c++) {
-      const double *p = d.ch_part + (size_t)c * 8 * d.n;
-      U += p[6 * (size_t)d.n + i];
-      V += p[7 * (size_t)d.n + i];
-    }
+  if (heavy) {
+    U = fix_get(d.acc_ch[8 * (size_t)i + 6]);
+    V = fix_get(d.acc_ch[8 * (size_t)i + 7]);
   }
-  const double vol = i < d.nh ? d.vols[i] : 1.0;
-  const double du = i < d.nh ? U / vol : 0.0, dv = i < d.nh ? V / vol : 0.0;
+  const double vol = heavy ? d.vols[i] : 1.0;
+  const double du = heavy ? U / vol : 0.0, dv = heavy ? V / vol : 0.0;
   const double f4 = 1.0 / (4.0 * kPi);
   d.deru[i] = du;
   d.derv[i] = dv;
-  d.derus[i] = d.df * f4 * d.psvol[i] * du;
+  d.derus[i] = S.df64 * f4 * d.psvol[i] * du;
   d.dervs[i] = f4 * d.psvol[i] * dv;
-  d.derh[i] = i < d.nh ? fix_get(d.acc_derh[i]) : 0.0;
+  d.derh[i] = heavy ? fix_get(d.acc_derh[i]) : 0.0;
   d.deru_f[i] = (float)d.deru[i];
   d.derv_f[i] = (float)d.derv[i];
   d.derus_f[i] = (float)d.derus[i];
@@ -175,20 +175,18 @@ __global__ __launch_bounds__(kEpi) void k_final(Dev d, double *sp, double *br, d
   const int i = blockIdx.x * kEpi + threadIdx.x;
   if (i >= d.n) return;
   const int e = d.int2ext[i];
-  const size_t n = d.n;
-  double g_gb[3] = {0, 0, 0}, g_vw[3] = {0, 0, 0};
-  for (int q = 0; q < 3; q++) g_gb[q] = 2.0 * d.df * fix_get(d.acc_gbg[3 * (size_t)i + q]);
+  const SysInfo &S = d.sys[d.asys[i]];
+  const bool heavy = i < S.a0 + S.nh;
+  const double df = S.df64;
+  double g_gb[3], g_vw[3];
   const double f4 = 1.0 / (4.0 * kPi);
-  for (int c = 0; c < d.nchunk_ch; c++) {
-    const double *p = d.ch_part + (size_t)c * 8 * n;
-    for (int q = 0; q < 3; q++) {
-      g_gb[q] += d.df * f4 * p[q * n + i];
-      g_vw[q] += f4 * p[(3 + q) * n + i];
-    }
+  for (int q = 0; q < 3; q++) {
+    g_gb[q] = 2.0 * df * fix_get(d.acc_gbg[3 * (size_t)i + q]) + df * f4 * fix_get(d.acc_ch[8 * (size_t)i + q]);
+    g_vw[q] = f4 * fix_get(d.acc_ch[8 * (size_t)i + 3 + q]);
   }
   double g_cav[3] = {0, 0, 0}, g_hb[3];
   for (int q = 0; q < 3; q++) g_hb[q] = fix_get(d.acc_hb[3 * (size_t)i + q]);
-  if (i < d.nh) {
+  if (heavy) {
     const long long *t = d.acc_grad + (size_t)i * TG_N;
     for (int q = 0; q < 3; q++) {
       g_gb[q] += fix_get(t[TG_GB + q]);
@@ -203,31 +201,34 @@ __global__ __launch_bounds__(kEpi) void k_final(Dev d, double *sp, double *br, d
     decav[3 * (size_t)e + q] = g_cav[q];
     dehb[3 * (size_t)e + q] = g_hb[q];
   }
-  sp[e] = i < d.nh ? d.sp[i] : 1.0;
+  sp[e] = heavy ? d.sp[i] : 1.0;
   br[e] = d.br[i];
-  sa[e] = i < d.nh ? d.sa_f[i] : 0.0;
+  sa[e] = heavy ? d.sa_f[i] : 0.0;
 }
 
-// fixed-order reduction of the per-block energy partials and site energies
-__global__ __launch_bounds__(256) void k_energy(Dev d, double *energies, int nscale, int nborn) {
+// fixed-order reduction of the per-block energy partials, GB tile pairs and
+// site energies; one workgroup per system, energies[7 * system + k]
+__global__ __launch_bounds__(256) void k_energy(Dev d, double *energies) {
   __shared__ double red[EP_COUNT][4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const SysInfo S = d.sys[blockIdx.x];
   double acc[EP_COUNT];
   for (int k = 0; k < EP_COUNT; k++) acc[k] = 0.0;
-  const int nb[EP_COUNT] = {nscale, nscale, nscale, nborn, nborn, nborn, 0, 0};
-  for (int k = 0; k < EP_PAIR; k++) {
-    // contiguous ranges per thread keep the order fixed
-    const int per = (nb[k] + 255) / 256;
-    for (int b = t * per; b < min(nb[k], (t + 1) * per); b++) acc[k] += d.epart[(size_t)k * d.epart_stride + b];
-  }
-  {
-    const int per = (d.ntiles + 255) / 256;
-    for (int b = t * per; b < min(d.ntiles, (t + 1) * per); b++) acc[EP_PAIR] += d.df * d.gb_epart[b];
-  }
-  {
-    const int per = (d.nws + 255) / 256;
-    for (int s = t * per; s < min(d.nws, (t + 1) * per); s++) acc[EP_HB] += d.ws_part[s];
-  }
+  // contiguous ranges per thread keep the order fixed
+  auto range = [&](int n, int &b0, int &b1) {
+    const int per = (n + 255) / 256;
+    b0 = min(n, t * per);
+    b1 = min(n, b0 + per);
+  };
+  int b0, b1;
+  range(S.nblk, b0, b1);
+  for (int k = 0; k < EP_PAIR; k++)
+    for (int b = b0; b < b1; b++) acc[k] += d.epart[(size_t)k * d.nablk + S.blk0 + b];
+  range(S.ntile, b0, b1);
+  // tile partials hold each unordered pair once; E_pair = 2 df sum_{i<j} q_i q_j / f
+  for (int b = b0; b < b1; b++) acc[EP_PAIR] += 2.0 * S.df64 * d.gb_epart[S.tile0 + b];
+  range(S.nws, b0, b1);
+  for (int b = b0; b < b1; b++) acc[EP_HB] += d.ws_part[S.ws0 + b];
   for (int k = 0; k < EP_COUNT; k++) {
     const double s = wave_sum(acc[k]);
     if (lane == 0) red[k][w] = s;
@@ -236,13 +237,14 @@ __global__ __launch_bounds__(256) void k_energy(Dev d, double *energies, int nsc
   if (t == 0) {
     double e[EP_COUNT];
     for (int k = 0; k < EP_COUNT; k++) e[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
-    energies[0] = kTokcal * (e[EP_SELF] + e[EP_PAIR]);
-    energies[1] = e[EP_VDW];
-    energies[2] = e[EP_CORR_VDW];
-    energies[3] = e[EP_CAV];
-    energies[4] = e[EP_CORR_CAV];
-    energies[5] = e[EP_HB];
-    energies[6] = e[EP_MOLVOL];
+    double *o = energies + 7 * (size_t)blockIdx.x;
+    o[0] = kTokcal * (e[EP_SELF] + e[EP_PAIR]);
+    o[1] = e[EP_VDW];
+    o[2] = e[EP_CORR_VDW];
+    o[3] = e[EP_CAV];
+    o[4] = e[EP_CORR_CAV];
+    o[5] = e[EP_HB];
+    o[6] = e[EP_MOLVOL];
   }
 }
 
@@ -250,7 +252,7 @@ void launch_final(const Dev &d, double *sp, double *br, double *sa, double *dgbd
                   double *dehb, double *energies, hipStream_t s) {
   hipLaunchKernelGGL(k_final, dim3((d.n + kEpi - 1) / kEpi), dim3(kEpi), 0, s, d, sp, br, sa, dgbdr, dvwdr, decav,
                      dehb);
-  hipLaunchKernelGGL(k_energy, dim3(1), dim3(256), 0, s, d, energies, scale_blocks(d), born_blocks(d));
+  hipLaunchKernelGGL(k_energy, dim3(d.nsys), dim3(256), 0, s, d, energies);
 }
 
 }  // namespace agb

@@ -262,7 +262,11 @@ __global__ __launch_bounds__(256) void k_ws_place(Dev d) {
       break;
     }
   }
-  for (int s = 0; s < def.nsite; s++) W[s].stale0 = d.ws_stale0[def.first + s];
+  const int sys = d.asys[A];
+  for (int s = 0; s < def.nsite; s++) {
+    W[s].stale0 = d.ws_stale0[def.first + s];
+    W[s].sys = sys;
+  }
 }
 
 void launch_ws_place(const Dev &d, hipStream_t s) {
@@ -288,7 +292,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
   const bool active = site < d.nws;
   WSite W;
   if (active) W = d.ws[site];
-  const GridParams g = *d.grid;
+  const int sys = active ? W.sys : 0;
+  const GridParams g = d.grid[sys];
+  const int *cstart = d.cell_start + (size_t)sys * (kMaxCells + 1);
   const float aw = kKFC / (kHbRadius * kHbRadius);
   int cnt = 0;
   if (active) {
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
           const int xx = cx + dx;
           if (xx < 0 || xx >= g.dim[0]) continue;
           const int c = (zz * g.dim[1] + yy) * g.dim[0] + xx;
-          const int beg = d.cell_start[c], end = d.cell_start[c + 1];
+          const int beg = cstart[c], end = cstart[c + 1];
           for (int base = beg; base < end; base += 64) {
             const int k = base + lane;
             bool pass = false;
@@ -379,7 +385,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
       if (vp > 0.f) {
         const int k = cs[w][e];
         const int idx = fixed ? k : stale;
-        const double sp = idx < d.n ? d.spe[idx] : 0.0;
+        const double sp = idx >= 0 ? d.spe[idx] : 0.0;
         freev -= sp * (double)vp;
         stale = k;
         ckeep[w][e] = 1;

@@ -2,75 +2,99 @@
 // energy with A_i and the constant-B gradient, and the chain rule at
 // constant self volumes fused with U_i / V_i.
 //
-// Layout: a block owns 256 rows (one per thread) and a chunk of columns
-// (gridDim.y chunks); columns are staged through LDS 256 at a time and
-// broadcast to the lanes.  Each row sums a tile in FP32 and tiles in FP64;
-// per-chunk row sums go to [chunk][row] partials reduced in a fixed order by
-// the epilogue kernels, so no atomics and bitwise-reproducible results.
-// For the spline kernels rows are grouped by radius type (row_perm, 256-row
-// blocks of one type), so a block stages only the tables its rows can use.
+// Work decomposition: each kernel has a list of work items, sorted by
+// decreasing size, and a queue head; a grid of persistent blocks takes items
+// until the list is exhausted, so systems of very different size (a batch)
+// share one launch with balanced load.
+//
+// Spline kernels (descreening, chain rule): an item is a block of rows of one
+// radius type and one system against a range of that system's columns; the
+// block stages the spline tables its rows can use in LDS, streams the columns
+// through LDS 256 at a time and each row (one per thread) sums a column tile
+// in FP32.  GB pairs: an item is a pair of 256-atom tiles of one system.
+// Row sums leave each item through order-independent fixed-point atomics,
+// so results are bitwise reproducible whatever the schedule.
 #include "agb_common.h"
 #include "agb_device.h"
 
 namespace agb {
 
-constexpr int kRows = 256;
 constexpr int kTile = 256;
+
+// stage the spline tables [t][*] (ROW_FIRST) or [*][t] of row type t
+template <int ROWS>
+__device__ __forceinline__ void stage_tables(const Dev &d, float2 *dst, int t, bool row_first) {
+  const int nt = d.ntypes;
+  for (int k = threadIdx.x; k < nt * kTblN; k += ROWS) {
+    const int tj = k / kTblN, node = k - tj * kTblN;
+    const int tab = row_first ? t * nt + tj : tj * nt + t;
+    dst[k] = d.tbl[(size_t)tab * kTblN + node];
+  }
+}
 
 // ------------------------------------------------------------ descreening
 // beta_i = 1/(R_i - 0.5) - (1/4pi) sum_{j heavy, j != i} sp_j Q4(d_ij; i <- j)
 // Q4 = f(d/R_j)/R_j from table [rtype_i][rtype_j]
 // (reference agbnp3_inverse_born_radii_nolist_soa, kernels.c:500-678)
-__global__ __launch_bounds__(kRows) void k_descreen(Dev d) {
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
   extern __shared__ float2 dyn_tab[];     // [ntypes][512]: tables [t_row][*]
   float2 (*tab)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab);
   __shared__ float4 cpos[kTile];     // x, y, z, 1/R
   __shared__ float2 cw[kTile];       // sp/R, rtype (as float bits)
-  const int b = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
-  const int t = d.block_type[b];
-  const int nt = d.ntypes;
-  for (int k = threadIdx.x; k < nt * kTblN; k += kRows) {
-    const int tj = k / kTblN, node = k - tj * kTblN;
-    tab[tj][node] = d.tbl[(size_t)(t * nt + tj) * kTblN + node];
-  }
-  const int i = d.row_perm[b * kRows + threadIdx.x];
-  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (i >= 0) pi = d.pos[i];
-  const int per = (d.nh + nchunk - 1) / nchunk;
-  const int c0 = chunk * per, c1 = min(d.nh, c0 + per);
+  __shared__ int slot;
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
-  double acc = 0.0;
-  for (int base = c0; base < c1; base += kTile) {
-    __syncthreads();
-    const int j = base + threadIdx.x;
-    if (j < c1) {
-      const float4 pj = d.pos[j];
-      const float iR = 1.0f / pj.w;
-      cpos[threadIdx.x] = make_float4(pj.x, pj.y, pj.z, iR);
-      cw[threadIdx.x] = make_float2(d.sp_f[j] * iR, __int_as_float(d.rtype[j]));
+  int staged = -1;
+  for (int it = next_item(&d.queue[Q_DESC], &slot); it < d.ndesc_items; it = next_item(&d.queue[Q_DESC], &slot)) {
+    const int4 item = d.desc_items[it];
+    const int t = d.rblk[item.x].x;
+    if (t != staged) {                 // uniform: every thread reads the same item
+      stage_tables<ROWS>(d, dyn_tab, t, true);
+      staged = t;
     }
-    __syncthreads();
-    const int nc = min(kTile, c1 - base);
-    float tacc = 0.f;
-    if (i >= 0) {
-      for (int c = 0; c < nc; c++) {
-        const float4 pj = cpos[c];
-        const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-        const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-        const float2 w = cw[c];
-        float q, dq;
-        q4_lookup<true, false>(tab[__float_as_int(w.y)], dist * pj.w, dx_, dxinv, dp1, dp2, 1.f, q, dq);
-        tacc += (base + c != i) ? q * w.x : 0.f;
+    const int i = d.row_perm[item.x * ROWS + threadIdx.x];
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i >= 0) pi = d.pos[i];
+    const int c0 = item.y, c1 = item.z;
+    double acc = 0.0;
+    for (int base = c0; base < c1; base += kTile) {
+      __syncthreads();
+      for (int k = threadIdx.x; k < kTile; k += ROWS) {
+        const int j = base + k;
+        if (j < c1) {
+          const float4 pj = d.pos[j];
+          const float iR = 1.0f / pj.w;
+          cpos[k] = make_float4(pj.x, pj.y, pj.z, iR);
+          cw[k] = make_float2(d.sp_f[j] * iR, __int_as_float(d.rtype[j]));
+        }
       }
+      __syncthreads();
+      const int nc = min(kTile, c1 - base);
+      float tacc = 0.f;
+      if (i >= 0) {
+        for (int c = 0; c < nc; c++) {
+          const float4 pj = cpos[c];
+          const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+          const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+          const float2 w = cw[c];
+          float q, dq;
+          q4_lookup<true, false>(tab[__float_as_int(w.y)], dist * pj.w, dx_, dxinv, dp1, dp2, 1.f, q, dq);
+          tacc += (base + c != i) ? q * w.x : 0.f;
+        }
+      }
+      acc += (double)tacc;
     }
-    acc += (double)tacc;
+    if (i >= 0) fix_add(&d.acc_beta[i], acc);
   }
-  if (i >= 0) d.beta_part[(size_t)chunk * d.n + i] = acc;
 }
 
-void launch_descreen(const Dev &d, hipStream_t s) {
+void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s) {
+  if (!d.ndesc_items) return;
   const size_t lds = (size_t)d.ntypes * kTblN * sizeof(float2);
-  hipLaunchKernelGGL(k_descreen, dim3(d.nrow_blocks, d.nchunk_desc), dim3(kRows), lds, s, d);
+  const int grid = t.desc_grid > 0 ? std::min(t.desc_grid, d.ndesc_items) : d.ndesc_items;
+  if (d.rows == 64) hipLaunchKernelGGL(k_descreen<64>, dim3(grid), dim3(64), lds, s, d);
+  else if (d.rows == 128) hipLaunchKernelGGL(k_descreen<128>, dim3(grid), dim3(128), lds, s, d);
+  else hipLaunchKernelGGL(k_descreen<256>, dim3(grid), dim3(256), lds, s, d);
 }
 
 // ------------------------------------------------------------- GB pairs
@@ -78,10 +102,10 @@ void launch_descreen(const Dev &d, hipStream_t s) {
 //   e   = q_i q_j / f,  f = sqrt(d2 + B_i B_j exp(-d2 / 4 B_i B_j))
 //   g_i += q_i q_j (1 - e/4) / f^3 (x_j - x_i),  g_j -= same      (x 2 df later)
 //   A_i, A_j += q_i q_j (B_i B_j + d2/4) exp(.) / f^3
-// A block takes a tile pair (I <= J) of 256-atom blocks.  Each wave holds 64
-// row atoms in its lanes and sweeps the 256 column atoms 64 at a time: the
-// column atom, its data and its partial sums move one lane per step with a
-// DPP wave rotation, so every lane meets every column once without LDS
+// An item is a tile pair of up to 256 x 256 atoms of one system.  Each wave
+// holds 64 row atoms in its lanes and sweeps the column atoms 64 at a time:
+// the column atom, its data and its partial sums move one lane per step with
+// a DPP wave rotation, so every lane meets every column once without LDS
 // traffic.  Diagonal tiles see each pair twice and keep only the row side.
 __device__ __forceinline__ float wave_rot(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
@@ -91,78 +115,87 @@ __device__ __forceinline__ int wave_rot(int v) { return __builtin_amdgcn_update_
 __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
   __shared__ float colacc[4][4][256];   // [wave][A, gx, gy, gz][column]: fixed-order sum
   __shared__ double ered[4];
-  const int2 tp = d.tile_ij[blockIdx.x];
-  const bool diag = tp.x == tp.y;
+  __shared__ int slot;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = tp.x * 256 + threadIdx.x;
-  float xi = 1e6f + (float)threadIdx.x, yi = 0.f, zi = 0.f, qi = 0.f, bi = 1.f;   // padding: far, no charge
-  if (i < d.n) {
-    const float4 p = d.pos[i];
-    xi = p.x; yi = p.y; zi = p.z;
-    qi = d.q[i];
-    bi = d.br_f[i];
-  }
-  float rA = 0.f, rx = 0.f, ry = 0.f, rz = 0.f, re = 0.f;
-  for (int c = 0; c < 4; c++) {
-    const int j0 = tp.y * 256 + c * 64 + lane;
-    float cx = -1e6f - (float)lane, cy = 0.f, cz = 0.f, cq = 0.f, cb = 1.f;
-    if (j0 < d.n) {
-      const float4 p = d.pos[j0];
-      cx = p.x; cy = p.y; cz = p.z;
-      cq = d.q[j0];
-      cb = d.br_f[j0];
+  for (int it = next_item(&d.queue[Q_GB], &slot); it < d.ngb_items; it = next_item(&d.queue[Q_GB], &slot)) {
+    const int4 tp = d.gb_items[it];      // rows [x, y), columns [z, w)
+    const bool diag = tp.x == tp.z;
+    const int i = tp.x + threadIdx.x;
+    const bool wave_rows = tp.x + w * 64 < tp.y;   // wave-uniform: any valid row
+    const int nch = (tp.w - tp.z + 63) >> 6;
+    float xi = 1e6f + (float)threadIdx.x, yi = 0.f, zi = 0.f, qi = 0.f, bi = 1.f;   // padding: far, no charge
+    if (i < tp.y) {
+      const float4 p = d.pos[i];
+      xi = p.x; yi = p.y; zi = p.z;
+      qi = d.q[i];
+      bi = d.br_f[i];
     }
-    int cj = j0;
-    float cA = 0.f, cgx = 0.f, cgy = 0.f, cgz = 0.f;
-    for (int s = 0; s < 64; s++) {
-      const float ex = cx - xi, ey = cy - yi, ez = cz - zi;
-      const float d2 = ex * ex + ey * ey + ez * ez;
-      const bool valid = (cj != i) && (d2 > 0.f);
-      const float qq = valid ? qi * cq : 0.f;
-      const float bb = bi * cb;
-      const float et = __expf(-0.25f * d2 * __builtin_amdgcn_rcpf(bb));
-      const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
-      const float fgb3 = fgb * fgb * fgb;
-      re += qq * fgb;
-      const float mw = qq * (1.f - 0.25f * et) * fgb3;
-      const float at = qq * (bb + 0.25f * d2) * et * fgb3;
-      rx += mw * ex; ry += mw * ey; rz += mw * ez;
-      rA += at;
-      cA += at;
-      cgx -= mw * ex; cgy -= mw * ey; cgz -= mw * ez;
-      cx = wave_rot(cx); cy = wave_rot(cy); cz = wave_rot(cz); cq = wave_rot(cq); cb = wave_rot(cb);
-      cj = wave_rot(cj);
-      cA = wave_rot(cA); cgx = wave_rot(cgx); cgy = wave_rot(cgy); cgz = wave_rot(cgz);
+    float rA = 0.f, rx = 0.f, ry = 0.f, rz = 0.f, re = 0.f;
+    for (int c = 0; c < nch; c++) {
+      float cA = 0.f, cgx = 0.f, cgy = 0.f, cgz = 0.f;
+      if (wave_rows) {
+        const int j0 = tp.z + c * 64 + lane;
+        float cx = -1e6f - (float)lane, cy = 0.f, cz = 0.f, cq = 0.f, cb = 1.f;
+        if (j0 < tp.w) {
+          const float4 p = d.pos[j0];
+          cx = p.x; cy = p.y; cz = p.z;
+          cq = d.q[j0];
+          cb = d.br_f[j0];
+        }
+        int cj = j0;
+        for (int s = 0; s < 64; s++) {
+          const float ex = cx - xi, ey = cy - yi, ez = cz - zi;
+          const float d2 = ex * ex + ey * ey + ez * ez;
+          const bool valid = (cj != i) && (d2 > 0.f);
+          const float qq = valid ? qi * cq : 0.f;
+          const float bb = bi * cb;
+          const float et = __expf(-0.25f * d2 * __builtin_amdgcn_rcpf(bb));
+          const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
+          const float fgb3 = fgb * fgb * fgb;
+          re += qq * fgb;
+          const float mw = qq * (1.f - 0.25f * et) * fgb3;
+          const float at = qq * (bb + 0.25f * d2) * et * fgb3;
+          rx += mw * ex; ry += mw * ey; rz += mw * ez;
+          rA += at;
+          cA += at;
+          cgx -= mw * ex; cgy -= mw * ey; cgz -= mw * ez;
+          cx = wave_rot(cx); cy = wave_rot(cy); cz = wave_rot(cz); cq = wave_rot(cq); cb = wave_rot(cb);
+          cj = wave_rot(cj);
+          cA = wave_rot(cA); cgx = wave_rot(cgx); cgy = wave_rot(cgy); cgz = wave_rot(cgz);
+        }
+        // 64 rotations bring every column's sums back to its starting lane
+      }
+      colacc[w][0][c * 64 + lane] = cA;
+      colacc[w][1][c * 64 + lane] = cgx;
+      colacc[w][2][c * 64 + lane] = cgy;
+      colacc[w][3][c * 64 + lane] = cgz;
     }
-    // 64 rotations bring every column's sums back to its starting lane
-    colacc[w][0][c * 64 + lane] = cA;
-    colacc[w][1][c * 64 + lane] = cgx;
-    colacc[w][2][c * 64 + lane] = cgy;
-    colacc[w][3][c * 64 + lane] = cgz;
-  }
-  const double e = wave_sum((double)re * (diag ? 0.5 : 1.0));
-  if (lane == 0) ered[w] = e;
-  if (i < d.n) {
-    fix_add(&d.acc_gbA[i], (double)rA);
-    fix_add(&d.acc_gbg[3 * (size_t)i], (double)rx);
-    fix_add(&d.acc_gbg[3 * (size_t)i + 1], (double)ry);
-    fix_add(&d.acc_gbg[3 * (size_t)i + 2], (double)rz);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) d.gb_epart[blockIdx.x] = ered[0] + ered[1] + ered[2] + ered[3];
-  const int j = tp.y * 256 + threadIdx.x;
-  if (!diag && j < d.n) {
-    const int t = threadIdx.x;
-    fix_add(&d.acc_gbA[j], (double)colacc[0][0][t] + colacc[1][0][t] + colacc[2][0][t] + colacc[3][0][t]);
+    const double e = wave_sum((double)re * (diag ? 0.5 : 1.0));
+    if (lane == 0) ered[w] = e;
+    if (i < tp.y) {
+      fix_add(&d.acc_gbA[i], (double)rA);
+      fix_add(&d.acc_gbg[3 * (size_t)i], (double)rx);
+      fix_add(&d.acc_gbg[3 * (size_t)i + 1], (double)ry);
+      fix_add(&d.acc_gbg[3 * (size_t)i + 2], (double)rz);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) d.gb_epart[d.gb_eslot[it]] = ered[0] + ered[1] + ered[2] + ered[3];
+    const int j = tp.z + threadIdx.x;
+    if (!diag && j < tp.w) {
+      const int t = threadIdx.x;
+      fix_add(&d.acc_gbA[j], (double)colacc[0][0][t] + colacc[1][0][t] + colacc[2][0][t] + colacc[3][0][t]);
 #pragma unroll
-    for (int q = 0; q < 3; q++)
-      fix_add(&d.acc_gbg[3 * (size_t)j + q],
-              (double)colacc[0][1 + q][t] + colacc[1][1 + q][t] + colacc[2][1 + q][t] + colacc[3][1 + q][t]);
+      for (int q = 0; q < 3; q++)
+        fix_add(&d.acc_gbg[3 * (size_t)j + q],
+                (double)colacc[0][1 + q][t] + colacc[1][1 + q][t] + colacc[2][1 + q][t] + colacc[3][1 + q][t]);
+    }
   }
 }
 
-void launch_gbpair(const Dev &d, hipStream_t s) {
-  if (d.ntiles) hipLaunchKernelGGL(k_gbpair, dim3(d.ntiles), dim3(256), 0, s, d);
+void launch_gbpair(const Dev &d, const Tuning &t, hipStream_t s) {
+  if (!d.ngb_items) return;
+  const int grid = t.gb_grid > 0 ? std::min(t.gb_grid, d.ngb_items) : d.ngb_items;
+  hipLaunchKernelGGL(k_gbpair, dim3(grid), dim3(256), 0, s, d);
 }
 
 // -------------------------------------------- chain rule at constant sp
@@ -173,103 +206,130 @@ void launch_gbpair(const Dev &d, hipStream_t s) {
 // abrw and 1/(4pi) for vdW; U_k = sum_j q2ab_j Q4(j<-k), V_k likewise with abrw.
 // (reference agbnp3_gb_ders_constvp_nolist_ps kernels.c:681-805 and
 //  agbnp3_gb_deruv_nolist_ps kernels.c:809-904, without the q4cache)
-__global__ __launch_bounds__(kRows) void k_chain(Dev d) {
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) void k_chain(Dev d) {
   extern __shared__ float2 dyn_tab[];              // [2][ntypes][512]
   float2 (*tab_kj)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab);                       // [t_k][*]
   float2 (*tab_jk)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab + (size_t)d.ntypes * kTblN);  // [*][t_k]
   __shared__ float4 cpos[kTile];                   // x, y, z, 1/R
   __shared__ float4 cpar[kTile];                   // sp, q2ab, abrw, rtype bits
-  const int b = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
-  const int t = d.block_type[b];
-  const int nt = d.ntypes;
-  for (int k = threadIdx.x; k < nt * kTblN; k += kRows) {
-    const int tj = k / kTblN, node = k - tj * kTblN;
-    tab_kj[tj][node] = d.tbl[(size_t)(t * nt + tj) * kTblN + node];
-    tab_jk[tj][node] = d.tbl[(size_t)(tj * nt + t) * kTblN + node];
-  }
-  const int kr = d.row_perm[b * kRows + threadIdx.x];
-  float4 pk = make_float4(0.f, 0.f, 0.f, 1.f);
-  float spk = 0.f, q2k = 0.f, abk = 0.f;
-  bool kheavy = false;
-  if (kr >= 0) {
-    pk = d.pos[kr];
-    q2k = d.q2ab_f[kr];
-    abk = d.abrw_f[kr];
-    kheavy = kr < d.nh;
-    spk = kheavy ? d.sp_f[kr] : 0.f;
-  }
-  const float iRk = 1.0f / pk.w;
-  // hydrogen rows only pair with heavy columns (heavy atoms come first);
-  // heavy and hydrogen rows of equal radius may share a block
-  const int ncol = d.block_hrows[b] ? d.nh : d.n;
-  const int per = (ncol + nchunk - 1) / nchunk;
-  const int c0 = chunk * per, c1 = min(ncol, c0 + per);
+  __shared__ int slot;
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
   const float sgn2 = (d.options & 2) ? -1.f : 1.f;
-  double gb[3] = {0, 0, 0}, vw[3] = {0, 0, 0}, U = 0.0, V = 0.0;
-  for (int base = c0; base < c1; base += kTile) {
-    __syncthreads();
-    const int j = base + threadIdx.x;
-    if (j < c1) {
-      const float4 pj = d.pos[j];
-      cpos[threadIdx.x] = make_float4(pj.x, pj.y, pj.z, 1.0f / pj.w);
-      cpar[threadIdx.x] = make_float4(j < d.nh ? d.sp_f[j] : 0.f, d.q2ab_f[j], d.abrw_f[j],
-                                      __int_as_float(d.rtype[j]));
+  int staged = -1;
+  for (int it = next_item(&d.queue[Q_CHAIN], &slot); it < d.nchain_items;
+       it = next_item(&d.queue[Q_CHAIN], &slot)) {
+    const int4 item = d.chain_items[it];
+    const int4 rb = d.rblk[item.x];
+    const int t = rb.x;
+    if (t != staged) {
+      stage_tables<ROWS>(d, dyn_tab, t, true);
+      stage_tables<ROWS>(d, dyn_tab + (size_t)d.ntypes * kTblN, t, false);
+      staged = t;
     }
-    __syncthreads();
-    const int nc = min(kTile, c1 - base);
-    float tg[3] = {0.f, 0.f, 0.f}, tv[3] = {0.f, 0.f, 0.f}, tU = 0.f, tV = 0.f;
+    const SysInfo &S = d.sys[rb.y];
+    const int hend = S.a0 + S.nh;                 // heavy atoms of the system end here
+    const int kr = d.row_perm[item.x * ROWS + threadIdx.x];
+    float4 pk = make_float4(0.f, 0.f, 0.f, 1.f);
+    float spk = 0.f, q2k = 0.f, abk = 0.f;
+    bool kheavy = false;
     if (kr >= 0) {
-      for (int c = 0; c < nc; c++) {
-        const int j = base + c;
-        const bool jheavy = j < d.nh;
-        if (j == kr || (!kheavy && !jheavy)) continue;
-        const float4 pj = cpos[c];
-        const float4 pp = cpar[c];
-        const float ex = pj.x - pk.x, ey = pj.y - pk.y, ez = pj.z - pk.z;
-        const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-        const int tj = __float_as_int(pp.w);
-        float hgb = 0.f, hvw = 0.f;
-        if (jheavy) {
-          float q, dq;
-          q4_lookup<false, true>(tab_kj[tj], dist * pj.w, dx_, dxinv, dp1, dp2, sgn2, q, dq);
-          dq *= pj.w * pj.w;
-          hgb += q2k * pp.x * dq;
-          hvw += abk * pp.x * dq;
-        }
-        if (kheavy) {
-          float q, dq;
-          q4_lookup<true, true>(tab_jk[tj], dist * iRk, dx_, dxinv, dp1, dp2, sgn2, q, dq);
-          q *= iRk;
-          dq *= iRk * iRk;
-          hgb += pp.y * spk * dq;
-          hvw += pp.z * spk * dq;
-          tU += pp.y * q;
-          tV += pp.z * q;
-        }
-        const float id = 1.0f / dist;
-        hgb *= id;
-        hvw *= id;
-        tg[0] += hgb * ex; tg[1] += hgb * ey; tg[2] += hgb * ez;
-        tv[0] += hvw * ex; tv[1] += hvw * ey; tv[2] += hvw * ez;
-      }
+      pk = d.pos[kr];
+      q2k = d.q2ab_f[kr];
+      abk = d.abrw_f[kr];
+      kheavy = kr < hend;
+      spk = kheavy ? d.sp_f[kr] : 0.f;
     }
-    for (int q = 0; q < 3; q++) { gb[q] += tg[q]; vw[q] += tv[q]; }
-    U += tU;
-    V += tV;
-  }
-  if (kr >= 0) {
-    double *p = d.ch_part + (size_t)chunk * 8 * d.n;
-    const size_t n = d.n;
-    p[kr] = gb[0]; p[n + kr] = gb[1]; p[2 * n + kr] = gb[2];
-    p[3 * n + kr] = vw[0]; p[4 * n + kr] = vw[1]; p[5 * n + kr] = vw[2];
-    p[6 * n + kr] = U; p[7 * n + kr] = V;
+    const float iRk = 1.0f / pk.w;
+    const int c0 = item.y, c1 = item.z;
+    double gb[3] = {0, 0, 0}, vw[3] = {0, 0, 0}, U = 0.0, V = 0.0;
+    for (int base = c0; base < c1; base += kTile) {
+      __syncthreads();
+      for (int k = threadIdx.x; k < kTile; k += ROWS) {
+        const int j = base + k;
+        if (j < c1) {
+          const float4 pj = d.pos[j];
+          cpos[k] = make_float4(pj.x, pj.y, pj.z, 1.0f / pj.w);
+          cpar[k] = make_float4(j < hend ? d.sp_f[j] : 0.f, d.q2ab_f[j], d.abrw_f[j], __int_as_float(d.rtype[j]));
+        }
+      }
+      __syncthreads();
+      const int nc = min(kTile, c1 - base);
+      float tg[3] = {0.f, 0.f, 0.f}, tv[3] = {0.f, 0.f, 0.f}, tU = 0.f, tV = 0.f;
+      if (kr >= 0) {
+        for (int c = 0; c < nc; c++) {
+          const int j = base + c;
+          const bool jheavy = j < hend;
+          if (j == kr || (!kheavy && !jheavy)) continue;
+          const float4 pj = cpos[c];
+          const float4 pp = cpar[c];
+          const float ex = pj.x - pk.x, ey = pj.y - pk.y, ez = pj.z - pk.z;
+          const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+          const int tj = __float_as_int(pp.w);
+          float hgb = 0.f, hvw = 0.f;
+          if (jheavy) {
+            float q, dq;
+            q4_lookup<false, true>(tab_kj[tj], dist * pj.w, dx_, dxinv, dp1, dp2, sgn2, q, dq);
+            dq *= pj.w * pj.w;
+            hgb += q2k * pp.x * dq;
+            hvw += abk * pp.x * dq;
+          }
+          if (kheavy) {
+            float q, dq;
+            q4_lookup<true, true>(tab_jk[tj], dist * iRk, dx_, dxinv, dp1, dp2, sgn2, q, dq);
+            q *= iRk;
+            dq *= iRk * iRk;
+            hgb += pp.y * spk * dq;
+            hvw += pp.z * spk * dq;
+            tU += pp.y * q;
+            tV += pp.z * q;
+          }
+          const float id = 1.0f / dist;
+          hgb *= id;
+          hvw *= id;
+          tg[0] += hgb * ex; tg[1] += hgb * ey; tg[2] += hgb * ez;
+          tv[0] += hvw * ex; tv[1] += hvw * ey; tv[2] += hvw * ez;
+        }
+      }
+      for (int q = 0; q < 3; q++) { gb[q] += tg[q]; vw[q] += tv[q]; }
+      U += tU;
+      V += tV;
+    }
+    if (kr >= 0) {
+      long long *p = d.acc_ch + 8 * (size_t)kr;
+      for (int q = 0; q < 3; q++) {
+        fix_add(p + q, gb[q]);
+        fix_add(p + 3 + q, vw[q]);
+      }
+      fix_add(p + 6, U);
+      fix_add(p + 7, V);
+    }
   }
 }
 
-void launch_chain(const Dev &d, hipStream_t s) {
+void launch_chain(const Dev &d, const Tuning &t, hipStream_t s) {
+  if (!d.nchain_items) return;
   const size_t lds = 2 * (size_t)d.ntypes * kTblN * sizeof(float2);
-  hipLaunchKernelGGL(k_chain, dim3(d.nrow_blocks, d.nchunk_ch), dim3(kRows), lds, s, d);
+  const int grid = t.chain_grid > 0 ? std::min(t.chain_grid, d.nchain_items) : d.nchain_items;
+  if (d.rows == 64) hipLaunchKernelGGL(k_chain<64>, dim3(grid), dim3(64), lds, s, d);
+  else if (d.rows == 128) hipLaunchKernelGGL(k_chain<128>, dim3(grid), dim3(128), lds, s, d);
+  else hipLaunchKernelGGL(k_chain<256>, dim3(grid), dim3(256), lds, s, d);
+}
+
+// allow the spline kernels the dynamic LDS their tables need
+hipError_t prepare_pairs(int ntypes) {
+  const int d1 = ntypes * kTblN * (int)sizeof(float2), d2 = 2 * d1;
+  hipError_t e = hipSuccess;
+  auto set = [&](const void *f, int bytes) {
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  };
+  set((const void *)k_descreen<64>, d1);
+  set((const void *)k_descreen<128>, d1);
+  set((const void *)k_descreen<256>, d1);
+  set((const void *)k_chain<64>, d2);
+  set((const void *)k_chain<128>, d2);
+  set((const void *)k_chain<256>, d2);
+  return e;
 }
 
 }  // namespace agb

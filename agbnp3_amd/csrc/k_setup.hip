@@ -1,5 +1,5 @@
-// Per-call setup kernels: coordinate ingest, cell grid, heavy-atom neighbour
-// list.
+// Per-call setup kernels: coordinate ingest, per-system cell grids, heavy-atom
+// neighbour list.
 //
 // The neighbour list must equal the reference's exactly (it fixes the shape
 // of the overlap tree): heavy j > i with
@@ -13,20 +13,16 @@ namespace agb {
 
 // ---------------------------------------------------------------- ingest
 // caller-order FP64 coordinates -> internal FP32 (x, y, z, R); clears the
-// scatter accumulators of the call; per-block bounding box of heavy atoms
+// scatter accumulators and work-queue heads of the call
 __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict__ x,
                                                 const double *__restrict__ y,
                                                 const double *__restrict__ z) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   if (i < d.n) {
     const int e = d.int2ext[i];
-    float4 p = make_float4((float)x[e], (float)y[e], (float)z[e], d.R[i]);
-    d.pos[i] = p;
-    if (i < d.nh) {
-      lo[0] = hi[0] = p.x;
-      lo[1] = hi[1] = p.y;
-      lo[2] = hi[2] = p.z;
+    d.pos[i] = make_float4((float)x[e], (float)y[e], (float)z[e], d.R[i]);
+    const SysInfo &S = d.sys[d.asys[i]];
+    if (i < S.a0 + S.nh) {
       d.acc_vol[i] = 0;
       d.acc_sa[i] = 0;
       d.acc_derh[i] = 0;
@@ -34,65 +30,71 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
       for (int k = 0; k < TG_N; k++) d.acc_grad[(size_t)i * TG_N + k] = 0;
     }
     d.acc_gbA[i] = 0;
+    d.acc_beta[i] = 0;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       d.acc_hb[3 * (size_t)i + q] = 0;
       d.acc_gbg[3 * (size_t)i + q] = 0;
     }
+#pragma unroll
+    for (int q = 0; q < 8; q++) d.acc_ch[8 * (size_t)i + q] = 0;
   }
   if (i == 0) {
     *d.status = 0;
     for (int k = 0; k < 4; k++) d.counters[k] = 0;
-  }
-  __shared__ float red[6][4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    float a = wave_min(lo[k]), b = wave_max(hi[k]);
-    if (lane == 0) { red[k][w] = a; red[3 + k][w] = b; }
-  }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    const int k = threadIdx.x;
-    float v = red[k][0];
-    for (int q = 1; q < 4; q++) v = k < 3 ? fminf(v, red[k][q]) : fmaxf(v, red[k][q]);
-    d.bbox_part[(size_t)k * gridDim.x + blockIdx.x] = v;
+    for (int k = 0; k < Q_COUNT; k++) d.queue[k] = 0;
+    if (d.skin > 0.f) {       // neighbour-list reuse: forced rebuilds
+      const int calls = d.nbl_state[1] + 1;
+      d.nbl_state[1] = calls;
+      d.nbl_state[2] = d.nbl_state[0] == 0 || (d.reuse_every > 0 && calls % d.reuse_every == 0);
+    }
   }
 }
 
 void launch_ingest(const Dev &d, const double *x, const double *y, const double *z, hipStream_t s) {
   const int nblk = (d.n + 255) / 256;
-  hipLaunchKernelGGL(k_ingest, dim3(nblk), dim3(256), 0, s, d, x, y, z);
+  if (nblk) hipLaunchKernelGGL(k_ingest, dim3(nblk), dim3(256), 0, s, d, x, y, z);
 }
 
 // ------------------------------------------------------------ cell grid
-// One workgroup: reduce the box, size the grid (cell edge >= largest
-// heavy-heavy cutoff, so a 27-cell stencil is complete), counting sort of
-// heavy atoms into cells.  kMaxCells bounds the LDS histogram; the edge grows
-// until the grid fits.
-constexpr int kMaxCells = 8192;
-
-__global__ __launch_bounds__(1024) void k_cells(Dev d, int nblk) {
+// One workgroup per system: bounding box of its heavy atoms, grid sizing
+// (cell edge >= largest heavy-heavy and site-heavy cutoff, so a 27-cell
+// stencil is complete), counting sort of the heavy atoms into cells.
+// kMaxCells bounds the LDS histogram; the edge grows until the grid fits.
+__global__ __launch_bounds__(1024) void k_cells(Dev d) {
   __shared__ int cnt[kMaxCells];
   __shared__ int wsum[16], wpre[16];
+  __shared__ float box[6][16];
   __shared__ GridParams g;
-  __shared__ float box[6];
-  const int t = threadIdx.x;
-  if (t < 6) {
-    float v = t < 3 ? INFINITY : -INFINITY;
-    for (int b = 0; b < nblk; b++) {
-      float u = d.bbox_part[(size_t)t * nblk + b];
-      v = t < 3 ? fminf(v, u) : fmaxf(v, u);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const SysInfo S = d.sys[blockIdx.x];
+  const int a0 = S.a0, nh = S.nh;
+  int *cstart = d.cell_start + (size_t)blockIdx.x * (kMaxCells + 1);
+  {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = t; k < nh; k += 1024) {
+      const float4 p = d.pos[a0 + k];
+      lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+      hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
     }
-    box[t] = v;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float a = wave_min(lo[k]), b = wave_max(hi[k]);
+      if (lane == 0) { box[k][w] = a; box[3 + k][w] = b; }
+    }
   }
   __syncthreads();
   if (t == 0) {
-    // every heavy-heavy and water-site-heavy cutoff is below 1.6 * 2 * Rmax
-    float cs = fmaxf(kNbOffset * 2.0f * d.rmax_heavy, kNbOffset * (d.rmax_heavy + kHbRadius)) * 1.001f + 1e-3f;
+    float bl[3], bh[3];
+    for (int k = 0; k < 3; k++) {
+      bl[k] = box[k][0];
+      bh[k] = box[3 + k][0];
+      for (int q = 1; q < 16; q++) { bl[k] = fminf(bl[k], box[k][q]); bh[k] = fmaxf(bh[k], box[3 + k][q]); }
+    }
+    float cs = fmaxf(kNbOffset * 2.0f * S.rmax_heavy, kNbOffset * (S.rmax_heavy + kHbRadius)) * 1.001f + 1e-3f;
     float ext[3];
     for (int k = 0; k < 3; k++) {
-      float lo = box[k], hi = box[k + 3];
+      float lo = bl[k], hi = bh[k];
       if (!(hi >= lo)) { lo = 0.f; hi = 0.f; }    // no heavy atoms
       g.lo[k] = lo - 1e-3f;
       ext[k] = hi - lo + 2e-3f;
@@ -104,33 +106,32 @@ __global__ __launch_bounds__(1024) void k_cells(Dev d, int nblk) {
         g.dim[k] = (int)floorf(ext[k] / cs) + 1;
         nc *= g.dim[k];
       }
-      if (nc <= kMaxCells && nc <= d.ncell_max) { g.ncell = (int)nc; break; }
+      if (nc <= kMaxCells) { g.ncell = (int)nc; break; }
       cs *= 1.05f;
     }
     g.cs = cs;
     g.inv_cs = 1.0f / cs;
-    *d.grid = g;
+    d.grid[blockIdx.x] = g;
   }
   __syncthreads();
   const int ncell = g.ncell;
   for (int c = t; c < ncell; c += 1024) cnt[c] = 0;
   __syncthreads();
-  for (int i = t; i < d.nh; i += 1024) {
-    float4 p = d.pos[i];
-    int c = cell_index(g, p.x, p.y, p.z);
-    d.cell_atoms[i] = c;            // temporary: cell of atom i
+  for (int k = t; k < nh; k += 1024) {
+    const float4 p = d.pos[a0 + k];
+    const int c = cell_index(g, p.x, p.y, p.z);
+    d.cell_tmp[a0 + k] = c;
     atomicAdd(&cnt[c], 1);
   }
   __syncthreads();
-  // exclusive scan of cnt over ncell cells: 8 cells per thread
+  // exclusive scan of cnt over the cells, contiguous cells per thread
   const int per = (ncell + 1023) / 1024;
   int local = 0;
   for (int k = 0; k < per; k++) {
-    int c = t * per + k;
+    const int c = t * per + k;
     if (c < ncell) local += cnt[c];
   }
-  int incl = wave_incl_scan(local);
-  const int lane = t & 63, w = t >> 6;
+  const int incl = wave_incl_scan(local);
   if (lane == 63) wsum[w] = incl;
   __syncthreads();
   if (t < 16) {
@@ -139,37 +140,40 @@ __global__ __launch_bounds__(1024) void k_cells(Dev d, int nblk) {
     wpre[t] = s;
   }
   __syncthreads();
-  int run = incl - local + wpre[w];
+  int run = a0 + incl - local + wpre[w];
   for (int k = 0; k < per; k++) {
-    int c = t * per + k;
+    const int c = t * per + k;
     if (c < ncell) {
-      int v = cnt[c];
-      d.cell_start[c] = run;
+      const int v = cnt[c];
+      cstart[c] = run;
       cnt[c] = run;                  // cursor
       run += v;
     }
   }
-  if (t == 0) d.cell_start[ncell] = d.nh;
+  if (t == 0) cstart[ncell] = a0 + nh;
   __syncthreads();
   // scatter; the order inside a cell is irrelevant (rows are re-sorted)
-  for (int i = t; i < d.nh; i += 1024) {
-    int c = d.cell_atoms[i];
-    int slot = atomicAdd(&cnt[c], 1);
-    d.cell_start[ncell + 1 + slot] = i;   // staged behind the starts
+  for (int k = t; k < nh; k += 1024) {
+    const int c = d.cell_tmp[a0 + k];
+    const int slot = atomicAdd(&cnt[c], 1);
+    d.cell_atoms[slot] = a0 + k;
   }
-  __syncthreads();
-  for (int k = t; k < d.nh; k += 1024) d.cell_atoms[k] = d.cell_start[ncell + 1 + k];
 }
 
 void launch_cells(const Dev &d, hipStream_t s) {
-  const int nblk = (d.n + 255) / 256;
-  hipLaunchKernelGGL(k_cells, dim3(1), dim3(1024), 0, s, d, nblk);
+  if (d.nsys) hipLaunchKernelGGL(k_cells, dim3(d.nsys), dim3(1024), 0, s, d);
 }
-void launch_grid(const Dev &, hipStream_t) {}
 
 // ------------------------------------------------------- neighbour list
 // one wavefront per heavy row; candidates from the 27-cell stencil are
-// compacted into LDS with ballots, then rank-sorted by (d2, j)
+// compacted into LDS with ballots, then rank-sorted by (d2, j).
+//
+// Reuse across calls (d.skin > 0): a rebuild also keeps, per row, every pair
+// within the cutoff plus the skin, and the positions it was built from.
+// While no heavy atom has moved more than skin/2 since, every pair inside the
+// cutoff is among those candidates, so later calls re-test and re-sort just
+// the candidates with the same FP32 arithmetic and get the same list as a
+// full rebuild.  k_guard decides on the device (graph replays included).
 constexpr int kNbWaves = 4;
 
 template <int CAP>
@@ -178,47 +182,96 @@ __global__ __launch_bounds__(64 * kNbWaves) void k_nblist(Dev d) {
   __shared__ float sd2[kNbWaves][CAP];
   __shared__ int sj[kNbWaves][CAP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = blockIdx.x * kNbWaves + w;
-  const bool active = i < d.nh;             // wave-uniform
-  const GridParams g = *d.grid;
-  const float4 pi = d.pos[active ? i : 0];
-  int cx, cy, cz;
-  cell_coords(g, pi.x, pi.y, pi.z, cx, cy, cz);
+  const int h = blockIdx.x * kNbWaves + w;
+  const bool active = h < d.nh;             // wave-uniform
+  const int i = active ? d.hrow[h] : 0;
+  const int sys = d.asys[i];
+  const float4 pi = d.pos[i];
+  const bool reuse = d.skin > 0.f;
+  const bool rebuild = !reuse || d.nbl_state[2] != 0;
   int cnt = 0;
-  for (int dz = -1; dz <= 1 && active; dz++) {
-    const int zz = cz + dz;
-    if (zz < 0 || zz >= g.dim[2]) continue;
-    for (int dy = -1; dy <= 1; dy++) {
-      const int yy = cy + dy;
-      if (yy < 0 || yy >= g.dim[1]) continue;
-      for (int dx = -1; dx <= 1; dx++) {
-        const int xx = cx + dx;
-        if (xx < 0 || xx >= g.dim[0]) continue;
-        const int c = (zz * g.dim[1] + yy) * g.dim[0] + xx;
-        const int beg = d.cell_start[c], end = d.cell_start[c + 1];
-        for (int base = beg; base < end; base += 64) {
-          const int k = base + lane;
-          bool pass = false;
-          int j = -1;
-          float d2 = 0.f;
-          if (k < end) {
-            j = d.cell_atoms[k];
-            if (j > i) {
-              const float4 pj = d.pos[j];
-              const float ex = __fsub_rn(pj.x, pi.x), ey = __fsub_rn(pj.y, pi.y), ez = __fsub_rn(pj.z, pi.z);
-              d2 = __fadd_rn(__fadd_rn(__fmul_rn(ex, ex), __fmul_rn(ey, ey)), __fmul_rn(ez, ez));
-              const float u = __fmul_rn(__fadd_rn(pi.w, pj.w), kNbOffset);
-              pass = d2 < __fmul_rn(u, u);
+  // exact test of pair (i, j); the candidate test (reuse) on the same d2
+  auto test = [&](int j, bool want_cand, bool &pass, bool &cpass, float &d2) {
+    const float4 pj = d.pos[j];
+    const float ex = __fsub_rn(pj.x, pi.x), ey = __fsub_rn(pj.y, pi.y), ez = __fsub_rn(pj.z, pi.z);
+    d2 = __fadd_rn(__fadd_rn(__fmul_rn(ex, ex), __fmul_rn(ey, ey)), __fmul_rn(ez, ez));
+    const float u = __fmul_rn(__fadd_rn(pi.w, pj.w), kNbOffset);
+    pass = d2 < __fmul_rn(u, u);
+    if (want_cand) {
+      const float uc = u + d.skin + 1e-3f;
+      cpass = d2 < uc * uc;
+    }
+  };
+  auto keep = [&](bool pass, int j, float d2) {
+    const unsigned long long m = __ballot(pass);
+    if (pass) {
+      const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
+      if (slot < CAP) { sd2[w][slot] = d2; sj[w][slot] = j; }
+    }
+    cnt += __popcll(m);
+  };
+  if (active && rebuild) {
+    const GridParams g = d.grid[sys];
+    const int *cstart = d.cell_start + (size_t)sys * (kMaxCells + 1);
+    int *cand = reuse ? d.cand_nbr + (size_t)h * kCandCap : nullptr;
+    int ccnt = 0;
+    int cx, cy, cz;
+    cell_coords(g, pi.x, pi.y, pi.z, cx, cy, cz);
+    for (int dz = -1; dz <= 1; dz++) {
+      const int zz = cz + dz;
+      if (zz < 0 || zz >= g.dim[2]) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        const int yy = cy + dy;
+        if (yy < 0 || yy >= g.dim[1]) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          const int xx = cx + dx;
+          if (xx < 0 || xx >= g.dim[0]) continue;
+          const int c = (zz * g.dim[1] + yy) * g.dim[0] + xx;
+          const int beg = cstart[c], end = cstart[c + 1];
+          for (int base = beg; base < end; base += 64) {
+            const int k = base + lane;
+            bool pass = false, cpass = false;
+            int j = -1;
+            float d2 = 0.f;
+            if (k < end) {
+              j = d.cell_atoms[k];
+              if (j > i) test(j, reuse, pass, cpass, d2);
+            }
+            keep(pass, j, d2);
+            if (reuse) {
+              const unsigned long long mc = __ballot(cpass);
+              if (cpass) {
+                const int slot = ccnt + __popcll(mc & ((1ull << lane) - 1ull));
+                if (slot < kCandCap) cand[slot] = j;
+              }
+              ccnt += __popcll(mc);
             }
           }
-          const unsigned long long m = __ballot(pass);
-          if (pass) {
-            const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (slot < CAP) { sd2[w][slot] = d2; sj[w][slot] = j; }
-          }
-          cnt += __popcll(m);
         }
       }
+    }
+    if (reuse && lane == 0) {
+      if (ccnt > kCandCap) atomicOr(d.status, ST_CAND_OVERFLOW);
+      d.cand_nne[h] = min(ccnt, kCandCap);
+      d.ref_pos[i] = pi;
+      if (h == 0) {
+        d.nbl_state[0] = 1;
+        d.nbl_state[3] += 1;       // rebuilds since reuse was set
+      }
+    }
+  } else if (active) {
+    const int nc = d.cand_nne[h];
+    const int *cand = d.cand_nbr + (size_t)h * kCandCap;
+    for (int base = 0; base < nc; base += 64) {
+      const int k = base + lane;
+      bool pass = false, cpass = false;
+      int j = -1;
+      float d2 = 0.f;
+      if (k < nc) {
+        j = cand[k];
+        test(j, false, pass, cpass, d2);
+      }
+      keep(pass, j, d2);
     }
   }
   int m = cnt;
@@ -236,19 +289,34 @@ __global__ __launch_bounds__(64 * kNbWaves) void k_nblist(Dev d) {
       const float df = sd2[w][f];
       rank += (df < de) || (df == de && sj[w][f] < je);
     }
-    d.nbr[(size_t)i * d.cap + rank] = je;
+    d.nbr[(size_t)h * d.cap + rank] = je;
   }
   if (lane == 0) {
-    d.nne[i] = m;
+    d.nne[h] = m;
     atomicAdd(&d.counters[2], (unsigned long long)m);
     atomicMax(&d.counters[3], (unsigned long long)m);
   }
 }
 
+// rebuild decision of the neighbour-list reuse: any heavy atom farther than
+// skin/2 from where the kept candidates were built
+__global__ __launch_bounds__(256) void k_guard(Dev d) {
+  const int h = blockIdx.x * 256 + threadIdx.x;
+  if (h >= d.nh || d.nbl_state[0] == 0) return;
+  const int i = d.hrow[h];
+  const float4 p = d.pos[i], r = d.ref_pos[i];
+  const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
+  const float half = 0.5f * d.skin;
+  if (!(ex * ex + ey * ey + ez * ez <= half * half)) d.nbl_state[2] = 1;   // also for non-finite input
+}
+
 void launch_nblist(const Dev &d, hipStream_t s) {
   const int nblk = (d.nh + kNbWaves - 1) / kNbWaves;
   if (nblk == 0) return;
-  if (d.cap <= 128)
+  if (d.skin > 0.f) hipLaunchKernelGGL(k_guard, dim3((d.nh + 255) / 256), dim3(256), 0, s, d);
+  if (d.cap <= 64)
+    hipLaunchKernelGGL(k_nblist<64>, dim3(nblk), dim3(64 * kNbWaves), 0, s, d);
+  else if (d.cap <= 128)
     hipLaunchKernelGGL(k_nblist<128>, dim3(nblk), dim3(64 * kNbWaves), 0, s, d);
   else
     hipLaunchKernelGGL(k_nblist<256>, dim3(nblk), dim3(64 * kNbWaves), 0, s, d);

@@ -219,11 +219,12 @@ template <int CAP, bool PB>
 __global__ __launch_bounds__(64) void k_tree(Dev d) {
   __shared__ TreeLds<CAP, PB> S;
   const int lane = threadIdx.x;
-  const int i = blockIdx.x;
-  const int m = d.nne[i];
+  const int h = blockIdx.x;
+  const int i = d.hrow[h];
+  const int m = d.nne[h];
   // stage the slot atoms
   for (int s = lane; s <= m; s += 64) {
-    const int at = s == 0 ? i : d.nbr[(size_t)i * d.cap + s - 1];
+    const int at = s == 0 ? i : d.nbr[(size_t)h * d.cap + s - 1];
     const float4 p = d.pos[at];
     S.idx[s] = at;
     S.a[s] = d.galpha[at];
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
 #pragma unroll
   for (int v = 0; v < TG_N; v++) root[v] = 0.0;
   unsigned nkept = 0, neval = 0;
-  const float df = d.eps_df;
+  const float df = d.sys[d.asys[i]].df;
   int path[4];
   path[0] = 0;
   const Gauss rg = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
@@ -288,7 +289,10 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
 
 void launch_tree(const Dev &d, int pass, hipStream_t s) {
   if (d.nh == 0) return;
-  if (d.cap <= 128) {
+  if (d.cap <= 64) {
+    if (pass == 0) hipLaunchKernelGGL((k_tree<64, false>), dim3(d.nh), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL((k_tree<64, true>), dim3(d.nh), dim3(64), 0, s, d);
+  } else if (d.cap <= 128) {
     if (pass == 0) hipLaunchKernelGGL((k_tree<128, false>), dim3(d.nh), dim3(64), 0, s, d);
     else hipLaunchKernelGGL((k_tree<128, true>), dim3(d.nh), dim3(64), 0, s, d);
   } else {

@@ -39,6 +39,69 @@ int agbnp3_ener_device(int tag, const double *x, const double *y, const double *
 int agbnp3_sync(int tag);
 void *agbnp3_stream(int tag);
 
+/* ---- neighbour-list reuse across calls ---------------------------------- */
+/* The reference rebuilds the heavy-atom neighbour list on every call
+   (agbnp3.c:3978-4115).  With skin > 0 (Angstrom, at most 4) a rebuild also
+   keeps every pair within the cutoff plus `skin`; later calls re-test and
+   re-sort only those candidates, which gives the identical list while no
+   heavy atom has moved more than skin/2 since the rebuild.  A device-side
+   guard checks that on every call and rebuilds when it fails; `every` > 0
+   also forces a rebuild every `every` calls.  skin = 0 restores a rebuild on
+   every call.  Results are bitwise those of the full rebuild. */
+int agbnp3_set_nblist_reuse(int tag, float skin, int every);
+/* calls and candidate rebuilds since the last agbnp3_set_nblist_reuse */
+int agbnp3_nblist_rebuilds(int tag, int *calls, int *rebuilds);
+int agbnp3_batch_set_nblist_reuse(int batch, float skin, int every);
+
+/* ---- launch shapes and autotuner ----------------------------------------- */
+/* params[7]: rows per spline-kernel block (64, 128, 256); columns per
+   descreening item and per chain-rule item (multiples of 256, 0 = automatic);
+   persistent grid of the descreening, GB-pair and chain-rule kernels (blocks,
+   0 = one per work item); neighbour-list row capacity (64, 128, 256). */
+#define AGBNP3_TUNING_PARAMS 7
+int agbnp3_get_tuning(int tag, int *params, int nparams);
+int agbnp3_set_tuning(int tag, const int *params, int nparams);
+/* times the pipeline on device coordinates x, y, z (caller order) for each
+   candidate shape, `reps` timed calls per trial, keeps the fastest per
+   kernel; a text report of the trials goes to `report` (may be NULL) */
+int agbnp3_tune(int tag, const double *x, const double *y, const double *z, int reps, char *report,
+                int report_len);
+
+/* ---- batches: many instances in one set of launches ---------------------- */
+/* A batch packs existing instances (same options) into one device arena;
+   every kernel of a call covers all of them, with per-system work items
+   taken from work queues by persistent blocks.  The batch copies what it
+   needs: the instances stay independent and may be deleted.  Arrays are the
+   per-instance caller-order arrays concatenated in `tags` order (offsets
+   from agbnp3_batch_offsets); energies are [ntags][7] as in
+   agbnp3_ener_device. */
+int agbnp3_batch_new(int *batch, int ntags, const int *tags);
+int agbnp3_batch_delete(int batch);
+/* nsys = ntags; offsets[ntags + 1]: first atom of each instance, total */
+int agbnp3_batch_offsets(int batch, int *nsys, int *offsets);
+int agbnp3_batch_set_options(int batch, int options);
+int agbnp3_batch_ener_device(int batch, const double *x, const double *y, const double *z, double *sp,
+                             double *br, double *surf_area, double *dgbdr, double *dvwdr, double *decav,
+                             double *dehb, double *energies);
+/* host arrays; out = sp, br, surf_area [n] each, dgbdr, dvwdr, decav, dehb
+   [3n] each, energies [ntags][7] (15 n + 7 ntags doubles) */
+int agbnp3_batch_ener(int batch, const double *x, const double *y, const double *z, double *out);
+int agbnp3_batch_sync(int batch);
+void *agbnp3_batch_stream(int batch);
+/* capture one agbnp3_batch_ener_device call on these device buffers as a
+   hipGraph; agbnp3_batch_graph_launch replays it on the batch stream (the
+   caller rewrites x, y, z between replays).  A change of launch shape or
+   options drops the graph. */
+int agbnp3_batch_graph_capture(int batch, const double *x, const double *y, const double *z, double *sp,
+                               double *br, double *surf_area, double *dgbdr, double *dvwdr, double *decav,
+                               double *dehb, double *energies);
+int agbnp3_batch_graph_launch(int batch);
+int agbnp3_batch_kernel_times(int batch, float *ms, int nmax, char *names, int names_len);
+int agbnp3_batch_get_tuning(int batch, int *params, int nparams);
+int agbnp3_batch_set_tuning(int batch, const int *params, int nparams);
+int agbnp3_batch_tune(int batch, const double *x, const double *y, const double *z, int reps, char *report,
+                      int report_len);
+
 /* ---- host-only helpers (no GPU needed) ------------------------------------ */
 /* Q4 spline tables for radius types `radii` (R = r + 0.5): y, y2 sized
    ntypes*ntypes*512, table (i, j) at offset (i*ntypes + j)*512; dx[0..1] =

@@ -31,6 +31,13 @@ def test_library_exports_every_declared_symbol():
     assert set(api.NBLIST_EXPORTED) == declared_functions("nblist.h")
 
 
+def test_single_precision_interface_flavour_exports_the_same_symbols():
+    api.load()
+    lib = ctypes.CDLL(_paths.LIBAGBNP3_SP)
+    for name in declared_functions("agbnp3.h") | declared_functions("agbnp3_amd.h"):
+        assert hasattr(lib, name), name
+
+
 def test_library_is_built_for_gfx950():
     # the offload bundle of our own build names its target triple
     blob = open(_paths.LIBAGBNP3, "rb").read()
@@ -65,3 +72,24 @@ def test_host_q4_tables_are_bit_identical_to_the_oracle():
         assert np.array_equal(yg, yo), case
         assert np.array_equal(y2g, y2o), case
         assert dx == np.float32(20.0) / np.float32(511)
+
+
+def test_distance_tests_are_compiled_without_contraction(tmp_path):
+    """The neighbour-list distance test must round like the reference's
+    separate FP32 multiplies and adds: the device code of the kernel may hold
+    no fused multiply-add (it has no other floating-point work)."""
+    import subprocess
+    from agbnp3_amd import build
+    for src, kernels in (("k_setup.hip", ("k_nblist",)),):
+        out = tmp_path / (src + ".s")
+        cmd = ([build.HIPCC] + build.HIP_FLAGS + build.HIP_FILE_FLAGS.get(src, []) +
+               [f"-I{_paths.INCLUDE_DIR}", f"-I{_paths.CSRC_DIR}", "--cuda-device-only", "-S",
+                os.path.join(_paths.CSRC_DIR, src), "-o", str(out)])
+        subprocess.run(cmd, check=True, capture_output=True)
+        asm = out.read_text()
+        for k in kernels:
+            bodies = [b for b in asm.split("\n_ZN3agb")[1:] if b.split("\n", 1)[0].lstrip("0123456789").startswith(k)]
+            assert bodies, (src, k)
+            for b in bodies:
+                body = b.split("s_endpgm", 1)[0]
+                assert "v_fma" not in body and "v_fmac" not in body, (src, k)

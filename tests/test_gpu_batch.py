@@ -1,0 +1,203 @@
+"""Batched evaluation through the C-ABI (agbnp3_batch_*): many instances of
+uneven size packed into one set of launches, the persistent work-queue
+kernels, hipGraph replay and the launch-shape autotuner.
+
+The bar: with equal launch shapes a batch is bitwise equal to evaluating each
+instance on its own (every sum is either a fixed-order FP64 reduction or an
+order-independent fixed-point one), graph replays are bitwise equal to eager
+calls, and a tuned batch stays within FP32 round-off of the untuned one.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from agbnp3_amd import api, hip, synth
+
+pytestmark = pytest.mark.gpu
+
+FIXED = dict(rows=256, desc_chunk=1024, chain_chunk=1024, desc_grid=0, gb_grid=0, chain_grid=0, cap=128)
+SEEDS = [(20262000 + k, n) for k, n in enumerate((300, 517, 1400, 2600))]
+OUTS = ("sp", "br", "surf_area", "dgbdr", "dvwdr", "decav", "dehb")
+
+
+@pytest.fixture(scope="module")
+def packed():
+    ss = [synth.generate(seed, n) for seed, n in SEEDS]
+    insts = [api.Instance(s) for s in ss]
+    for i in insts:
+        i.set_tuning(**FIXED)
+    b = api.Batch(insts)
+    b.set_tuning(**FIXED)
+    yield ss, insts, b
+    b.delete()
+    for i in insts:
+        i.delete()
+
+
+def concat(ss, k):
+    return np.concatenate([getattr(s, k) for s in ss])
+
+
+class DeviceIO:
+    """Device input and output buffers of one batch call."""
+
+    def __init__(self, n, nsys):
+        self.n, self.nsys = n, nsys
+        self.xyz = [hip.DeviceArray(8 * n) for _ in range(3)]
+        self.out = [hip.DeviceArray(8 * n) for _ in range(3)] + [hip.DeviceArray(24 * n) for _ in range(4)]
+        self.e = hip.DeviceArray(56 * nsys)
+
+    def ptrs(self):
+        return [a.addr for a in self.xyz] + [a.addr for a in self.out] + [self.e.addr]
+
+    def set_coords(self, x, y, z):
+        for d, a in zip(self.xyz, (x, y, z)):
+            hip.check(hip.lib().hipMemcpy(d.ptr, np.ascontiguousarray(a, np.float64).ctypes.data, 8 * self.n,
+                                          hip.H2D), "hipMemcpy")
+
+    def read(self):
+        n = self.n
+        r = {k: self.out[i].to_numpy(np.float64, (n,)) for i, k in enumerate(OUTS[:3])}
+        r.update({k: self.out[3 + i].to_numpy(np.float64, (n, 3)) for i, k in enumerate(OUTS[3:])})
+        r["energies"] = self.e.to_numpy(np.float64, (self.nsys, 7))
+        return r
+
+    def free(self):
+        for a in self.xyz + self.out + [self.e]:
+            a.free()
+
+
+def test_batch_equals_each_instance_bitwise(packed):
+    ss, insts, b = packed
+    out = b.ener(concat(ss, "x"), concat(ss, "y"), concat(ss, "z"))
+    for k, inst in enumerate(insts):
+        ref = inst.ener()
+        o0, o1 = b.offsets[k], b.offsets[k + 1]
+        for name in OUTS:
+            assert np.array_equal(out[name][o0:o1], ref[name]), (k, name)
+        assert np.array_equal(out["energies"][k], [ref[t] for t in api.ENERGY_NAMES]), k
+
+
+def test_batch_member_matches_fp64_oracle(packed):
+    ss, insts, b = packed
+    out = b.ener(concat(ss, "x"), concat(ss, "y"), concat(ss, "z"))
+    k = 2
+    ref = oracle.Oracle(ss[k]).ener()
+    o0, o1 = b.offsets[k], b.offsets[k + 1]
+    for i, t in enumerate(api.ENERGY_NAMES):
+        if ref[t] == 0.0:
+            assert abs(out["energies"][k][i]) < 1e-9
+        else:
+            assert abs(out["energies"][k][i] - ref[t]) <= 1e-5 * abs(ref[t]), t
+    for g in ("dgbdr", "dvwdr", "decav", "dehb"):
+        assert np.sqrt(np.mean((out[g][o0:o1] - ref[g]) ** 2)) <= 1e-4, g
+
+
+def test_persistent_grids_do_not_change_results(packed):
+    ss, insts, b = packed
+    x, y, z = concat(ss, "x"), concat(ss, "y"), concat(ss, "z")
+    ref = b.ener(x, y, z)
+    try:
+        b.set_tuning(desc_grid=7, gb_grid=5, chain_grid=3)
+        out = b.ener(x, y, z)
+    finally:
+        b.set_tuning(**FIXED)
+    for name in OUTS + ("energies",):
+        assert np.array_equal(out[name], ref[name]), name
+
+
+def test_row_block_shapes_agree(packed):
+    ss, insts, b = packed
+    x, y, z = concat(ss, "x"), concat(ss, "y"), concat(ss, "z")
+    ref = b.ener(x, y, z)
+    try:
+        for rows in (64, 128):
+            b.set_tuning(rows=rows)
+            out = b.ener(x, y, z)
+            # a row's column tiles and their FP32 sums do not depend on how rows are grouped
+            for name in OUTS + ("energies",):
+                assert np.array_equal(out[name], ref[name]), (rows, name)
+    finally:
+        b.set_tuning(**FIXED)
+
+
+def test_graph_replay_equals_eager_calls(packed):
+    ss, insts, b = packed
+    n = b.n
+    x0, y0, z0 = concat(ss, "x"), concat(ss, "y"), concat(ss, "z")
+    g_io, e_io = DeviceIO(n, b.nsys), DeviceIO(n, b.nsys)
+    try:
+        g_io.set_coords(x0, y0, z0)
+        b.graph_capture(*g_io.ptrs())
+        for r in range(3):
+            rng = np.random.default_rng(20263000 + r)
+            x, y, z = (a + rng.normal(0.0, 0.05, n) for a in (x0, y0, z0))
+            g_io.set_coords(x, y, z)
+            assert b.graph_launch() == api.AGBNP_OK
+            assert b.sync() == api.AGBNP_OK
+            got = g_io.read()
+            e_io.set_coords(x, y, z)
+            assert b.ener_device(*e_io.ptrs()) == api.AGBNP_OK
+            assert b.sync() == api.AGBNP_OK
+            want = e_io.read()
+            for name in OUTS + ("energies",):
+                assert np.array_equal(got[name], want[name]), (r, name)
+    finally:
+        g_io.free()
+        e_io.free()
+
+
+def test_autotuner_keeps_results(packed):
+    ss, insts, b = packed
+    n = b.n
+    x, y, z = concat(ss, "x"), concat(ss, "y"), concat(ss, "z")
+    ref = b.ener(x, y, z)
+    io = DeviceIO(n, b.nsys)
+    try:
+        io.set_coords(x, y, z)
+        report = b.tune(io.xyz[0].addr, io.xyz[1].addr, io.xyz[2].addr, reps=1)
+        assert "chosen" in report
+        t = b.get_tuning()
+        assert t["rows"] in (64, 128, 256)
+        out = b.ener(x, y, z)
+        for name in OUTS:
+            assert np.allclose(out[name], ref[name], rtol=1e-6, atol=1e-7), name
+        assert np.allclose(out["energies"], ref["energies"], rtol=1e-7, atol=1e-8)
+    finally:
+        io.free()
+        b.set_tuning(**FIXED)
+
+
+def test_batch_rejects_mixed_options():
+    s1, s2 = synth.generate(20262100, 300), synth.generate(20262101, 300)
+    a, c = api.Instance(s1), api.Instance(s2, options=api.OPT_HB_FIXED)
+    try:
+        with pytest.raises(RuntimeError):
+            api.Batch([a, c])
+    finally:
+        a.delete()
+        c.delete()
+
+
+def test_config2_shaped_batch():
+    """64 instances with log-uniform sizes in [300, 6000] (the batched
+    workload's size law), each member bitwise equal to its own evaluation."""
+    sizes = synth.sizes(20262999, 64, 300, 6000)
+    ss = [synth.generate(20262000 + k, int(n)) for k, n in enumerate(sizes)]
+    insts = [api.Instance(s) for s in ss]
+    for i in insts:
+        i.set_tuning(**FIXED)
+    b = api.Batch(insts)
+    try:
+        b.set_tuning(**FIXED)
+        out = b.ener(concat(ss, "x"), concat(ss, "y"), concat(ss, "z"))
+        for k in (0, 17, 40, 63):
+            ref = insts[k].ener()
+            o0, o1 = b.offsets[k], b.offsets[k + 1]
+            for name in OUTS:
+                assert np.array_equal(out[name][o0:o1], ref[name]), (k, name)
+            assert np.array_equal(out["energies"][k], [ref[t] for t in api.ENERGY_NAMES]), k
+    finally:
+        b.delete()
+        for i in insts:
+            i.delete()

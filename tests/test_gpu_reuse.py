@@ -1,0 +1,73 @@
+"""Neighbour-list reuse across calls (agbnp3_set_nblist_reuse): a trajectory
+of small seeded displacements evaluated with kept candidates must give
+bitwise the results of rebuilding the list on every call, including calls
+where the displacement guard or the forced period triggers a rebuild."""
+import numpy as np
+import pytest
+
+from agbnp3_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+OUTS = ("sp", "br", "surf_area", "dgbdr", "dvwdr", "decav", "dehb") + api.ENERGY_NAMES
+
+
+def frames(s, nframes, sigma, seed0):
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    for f in range(nframes):
+        rng = np.random.default_rng(seed0 + f)
+        x, y, z = (a + rng.normal(0.0, sigma, a.size) for a in (x, y, z))
+        yield x, y, z
+
+
+def same(a, b):
+    for k in OUTS:
+        if not np.array_equal(np.asarray(a[k]), np.asarray(b[k])):
+            return k
+    return None
+
+
+def test_reuse_equals_rebuild_on_a_trajectory():
+    s = synth.generate(20266150, 3000)
+    plain, reuse = api.Instance(s), api.Instance(s)
+    try:
+        reuse.set_nblist_reuse(1.0)
+        for f, (x, y, z) in enumerate(frames(s, 12, 0.03, 20267000)):
+            bad = same(reuse.ener(x, y, z), plain.ener(x, y, z))
+            assert bad is None, (f, bad)
+        calls, rebuilds = reuse.nblist_rebuilds()
+        assert calls == 12
+        assert 1 <= rebuilds < calls          # candidates were kept across calls
+    finally:
+        plain.delete()
+        reuse.delete()
+
+
+def test_guard_rebuilds_after_a_large_move():
+    s = synth.generate(20266151, 1500)
+    plain, reuse = api.Instance(s), api.Instance(s)
+    try:
+        reuse.set_nblist_reuse(0.8)
+        reuse.ener()
+        x = s.x.copy()
+        x[:200] += 0.6                      # beyond skin / 2 for the first heavy atoms
+        assert same(reuse.ener(x, s.y, s.z), plain.ener(x, s.y, s.z)) is None
+        calls, rebuilds = reuse.nblist_rebuilds()
+        assert (calls, rebuilds) == (2, 2)
+    finally:
+        plain.delete()
+        reuse.delete()
+
+
+def test_forced_rebuild_period():
+    s = synth.generate(20266152, 1000)
+    plain, reuse = api.Instance(s), api.Instance(s)
+    try:
+        reuse.set_nblist_reuse(1.0, every=3)
+        for f, (x, y, z) in enumerate(frames(s, 7, 0.01, 20267100)):
+            assert same(reuse.ener(x, y, z), plain.ener(x, y, z)) is None, f
+        calls, rebuilds = reuse.nblist_rebuilds()
+        assert calls == 7 and rebuilds == 3      # calls 1 (first), 3 and 6
+    finally:
+        plain.delete()
+        reuse.delete()

@@ -1,0 +1,63 @@
+"""The single-precision interface flavour (agbnp3.h with SINGLE_PREC_INTERFACE,
+float_i = float; reference agbnp3.h:43-50) against the double flavour: on
+inputs that are exactly representable in FP32 the engine sees identical
+values, so every output is the double flavour's output rounded to float."""
+import ctypes
+import dataclasses
+
+import numpy as np
+import pytest
+
+from agbnp3_amd import _paths, api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def load_sp():
+    api.load()
+    L = ctypes.CDLL(_paths.LIBAGBNP3_SP)
+    vp, ci, fp = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    L.agbnp3_new.argtypes = [ctypes.POINTER(ci), ci, vp, vp, vp, vp, vp, fp, fp, vp, vp, vp, vp, vp, vp, ci, vp,
+                             ctypes.POINTER(api.NeighList), ci]
+    L.agbnp3_ener.argtypes = [ci, ci] + [vp] * 17
+    L.agbnp3_delete.argtypes = [ci]
+    return L
+
+
+def test_float_interface_equals_double_interface_rounded():
+    s = synth.generate(20260001, 1200)
+    f32 = {k: np.asarray(getattr(s, k), np.float32) for k in ("x", "y", "z", "r", "charge", "igamma", "sgamma",
+                                                               "ialpha", "salpha", "hbcorr")}
+    s64 = dataclasses.replace(s, **{k: v.astype(np.float64) for k, v in f32.items()})
+    dp = api.Instance(s64)
+    ref = dp.ener()
+    L = load_sp()
+    assert L.agbnp3_initialize() == api.AGBNP_OK
+    conn = api.make_conntbl(s.conn_nne, s.conn_neighl1)
+    hb = np.ascontiguousarray(s.hbtype, np.int32)
+    ih = np.ascontiguousarray(s.ihydrogen, np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    tag = ctypes.c_int(-1)
+    rc = L.agbnp3_new(ctypes.byref(tag), s.natoms, p(f32["x"]), p(f32["y"]), p(f32["z"]), p(f32["r"]),
+                      p(f32["charge"]), float(s.eps_in), float(s.eps_out), p(f32["igamma"]), p(f32["sgamma"]),
+                      p(f32["ialpha"]), p(f32["salpha"]), p(hb), p(f32["hbcorr"]), int(s.nhydrogen), p(ih),
+                      ctypes.byref(conn), 0)
+    assert rc == api.AGBNP_OK
+    n = s.natoms
+    sp, br, sa = (np.zeros(n, np.float32) for _ in range(3))
+    g = [np.zeros((n, 3), np.float32) for _ in range(4)]
+    e = [ctypes.c_float(0.0) for _ in range(7)]
+    try:
+        rc = L.agbnp3_ener(tag.value, 0, p(f32["x"]), p(f32["y"]), p(f32["z"]), p(sp), p(br), ctypes.byref(e[6]),
+                           p(sa), ctypes.byref(e[0]), p(g[0]), ctypes.byref(e[1]), ctypes.byref(e[2]), p(g[1]),
+                           ctypes.byref(e[3]), ctypes.byref(e[4]), p(g[2]), ctypes.byref(e[5]), p(g[3]))
+        assert rc == api.AGBNP_OK
+        for k, v in zip(api.ENERGY_NAMES, e):
+            assert v.value == np.float32(ref[k]), k
+        for k, a in (("sp", sp), ("br", br), ("surf_area", sa), ("dgbdr", g[0]), ("dvwdr", g[1]),
+                     ("decav", g[2]), ("dehb", g[3])):
+            assert np.array_equal(a, ref[k].astype(np.float32)), k
+    finally:
+        L.agbnp3_delete(tag.value)
+        dp.delete()
+        api._nb.nblist_delete_neighbor_list(ctypes.byref(conn))
