@@ -16,10 +16,13 @@ def lib():
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipGetErrorString.restype = ctypes.c_char_p
+        _hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                        ctypes.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
     return _hip
 
 
-H2D, D2H = 1, 2
+H2D, D2H, D2D = 1, 2, 3
 
 
 def check(rc, what):
@@ -57,3 +60,12 @@ class DeviceArray:
 
 def synchronize():
     check(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def copy_async(dst, src, nbytes, kind, stream):
+    """hipMemcpyAsync between raw addresses on `stream`."""
+    check(lib().hipMemcpyAsync(dst, src, int(nbytes), kind, stream), "hipMemcpyAsync")
+
+
+def stream_synchronize(stream):
+    check(lib().hipStreamSynchronize(stream), "hipStreamSynchronize")

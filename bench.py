@@ -12,7 +12,13 @@ on the instance stream during the timed steps) and the reference CPU path
 (the reference library built from its own sources, OpenMP) timed on this
 host on a bounded sample of the same workload.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Batched form (config 2), reported beside: 512 instances of log-uniform size
+in [300, 6000] atoms (sizes seed 20262999, instance k seed 20262000 + k),
+evaluated as one batch replayed through a hipGraph after per-instance
+Gaussian coordinate perturbations (sigma 0.05 A, replay r seed 20263000 + r),
+against the same instances evaluated one agbnp3_ener_device call each.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched]
 """
 import argparse
 import json
@@ -63,6 +69,95 @@ def cpu_baseline(system, calls, threads):
                 sample=f"1 oracle evaluation of the same {system.natoms}-atom solute, {dt:.3f} s")
 
 
+def bench_batched(args, api, hip, synth, np):
+    """Config 2: plain per-instance calls vs one batch replayed by hipGraph."""
+    sizes = synth.sizes(20262999, args.batch_instances, 300, 6000)
+    ss = [synth.generate(20262000 + k, int(nk)) for k, nk in enumerate(sizes)]
+    log(f"bench: batched workload, {len(ss)} instances, {int(sizes.sum())} atoms")
+    insts = [api.Instance(s) for s in ss]
+    b = api.Batch(insts)
+    n = b.n
+    off = b.offsets
+    reps = args.batch_steps + args.warmup
+    base = [np.concatenate([getattr(s, k) for s in ss]) for k in ("x", "y", "z")]
+    # perturbed coordinates of every replay, resident in HBM before timing
+    frames = []
+    for r in range(reps):
+        rng = np.random.default_rng(20263000 + r)
+        frames.append([hip.DeviceArray.from_numpy(a + rng.normal(0.0, 0.05, n)) for a in base])
+    xyz = [hip.DeviceArray(8 * n) for _ in range(3)]
+    outs = [hip.DeviceArray(8 * n) for _ in range(3)] + [hip.DeviceArray(24 * n) for _ in range(4)]
+    en = hip.DeviceArray(56 * b.nsys)
+    ptrs = [a.addr for a in xyz] + [a.addr for a in outs] + [en.addr]
+    stream = b.stream()
+    for k in range(3):
+        hip.copy_async(xyz[k].addr, frames[0][k].addr, 8 * n, hip.D2D, stream)
+    report = b.tune(xyz[0].addr, xyz[1].addr, xyz[2].addr, reps=2)
+    tuning = b.get_tuning()
+    log("bench: batch tuning " + json.dumps(tuning))
+    b.graph_capture(*ptrs)
+
+    def replay(r):
+        for k in range(3):
+            hip.copy_async(xyz[k].addr, frames[r][k].addr, 8 * n, hip.D2D, stream)
+        if b.graph_launch() != api.AGBNP_OK:
+            raise RuntimeError("graph launch failed")
+
+    for r in range(args.warmup):
+        replay(r)
+    if b.sync() != api.AGBNP_OK:
+        raise RuntimeError("batch status error")
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for r in range(args.warmup, reps):
+        replay(r)
+    rc = b.sync()
+    hip.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.batch_steps
+    if rc != api.AGBNP_OK:
+        raise RuntimeError("batch status error in timed replays")
+    e_graph = en.to_numpy(np.float64, (b.nsys, 7))
+    # plain form: one agbnp3_ener_device per instance, same device inputs
+    o_sp = [a.addr for a in outs]
+
+    def plain(r):
+        for k, inst in enumerate(insts):
+            o = int(off[k])
+            args_k = [frames[r][q].addr + 8 * o for q in range(3)] + [o_sp[q] + 8 * o for q in range(3)] + \
+                     [o_sp[3 + q] + 24 * o for q in range(4)] + [en.addr + 56 * k]
+            if inst.ener_device(*args_k) != api.AGBNP_OK:
+                raise RuntimeError("agbnp3_ener_device failed")
+
+    for r in range(args.warmup):
+        plain(r)
+    for inst in insts:
+        if inst.sync() != api.AGBNP_OK:
+            raise RuntimeError("instance status error")
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for r in range(args.warmup, reps):
+        plain(r)
+    for inst in insts:
+        inst.sync()
+    hip.synchronize()
+    t_plain = (time.perf_counter() - t0) / args.batch_steps
+    e_plain = en.to_numpy(np.float64, (b.nsys, 7))
+    # both forms on the last replay's coordinates (tuned shapes group FP32 tile sums differently)
+    rel = float(np.max(np.abs(e_plain - e_graph) / np.maximum(np.abs(e_plain), 1e-6)))
+    ktimes = b.kernel_times()
+    for a in xyz + outs + [en] + [a for f in frames for a in f]:
+        a.free()
+    b.delete()
+    for inst in insts:
+        inst.delete()
+    return dict(workload=f"config 2: {len(ss)} instances, sizes log-uniform in [300, 6000] (seed 20262999), "
+                         f"{n} atoms per replay, hipGraph replay after sigma 0.05 A perturbations",
+                natoms=int(n), instances=len(ss), replays=args.batch_steps,
+                batched_atom_evals_per_s=n / t_batch, batched_ms_per_replay=1e3 * t_batch,
+                plain_atom_evals_per_s=n / t_plain, plain_ms_per_replay=1e3 * t_plain,
+                speedup=t_plain / t_batch, max_rel_energy_diff_vs_plain=rel, tuning=tuning)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +167,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20260025)
     ap.add_argument("--cpu-calls", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build evaluates one instance on one GPU; --gpus {args.gpus} is not supported "
@@ -140,6 +238,12 @@ def main():
                     avg_ms=timed[dom], flops_per_launch=flops[dom])
     total_flops = sum(flops.values())
 
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(args, api, hip, synth, np)
+        log(f"bench: batched {batched['batched_ms_per_replay']:.2f} ms/replay, plain loop "
+            f"{batched['plain_ms_per_replay']:.2f} ms, speedup {batched['speedup']:.2f}x")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -157,6 +261,7 @@ def main():
         pair_tflops=total_flops / (ms_per_step * 1e-3) / 1e12,
         host_api_atom_evals_per_s=host_rate,
         kernel_ms=ktimes,
+        batched=batched,
     )
     print(json.dumps(line), flush=True)
 
