@@ -1,0 +1,116 @@
+"""Pins the CPU reference of the Born-radius surrogate (oracle/surrogate.py)
+by mathematics: B-spline partition of unity and derivatives, descriptor
+gradients and MLP gradients against central finite differences, bf16
+rounding against the IEEE definition, and descriptors against a brute-force
+double loop."""
+import numpy as np
+
+from oracle import surrogate as sg
+
+
+def test_bf16_rounding_matches_ieee_round_to_nearest_even():
+    x = np.array([1.0, 1.00390625, 1.005859375, -3.14159, 65504.0, 1e-3], np.float32)
+    r = sg.bf16(x)
+    # bf16 keeps 8 significant bits; the result is the nearest, ties to even
+    for v, b in zip(x.astype(np.float64), r):
+        m, e = np.frexp(v)
+        grid = np.ldexp(np.round(m * 256.0), e - 8)
+        lo, hi = np.ldexp(np.floor(m * 256.0), e - 8), np.ldexp(np.ceil(m * 256.0), e - 8)
+        assert b in (lo, hi) and abs(b - v) <= abs(grid - v) + 1e-30
+    assert sg.bf16(np.float32(1.00390625)) == 1.0          # tie -> even mantissa
+
+
+def test_bspline_basis_partition_and_derivative():
+    d = np.linspace(1.6, 10.0, 500)                          # inside the shells, below the taper
+    w, wp = sg.shell_weights(d)
+    assert np.allclose(w.sum(1), 1.0, atol=1e-12)
+    assert np.allclose(wp.sum(1), 0.0, atol=1e-12)
+    h = 1e-6
+    wpl, _ = sg.shell_weights(d - h)
+    wph, _ = sg.shell_weights(d + h)
+    assert np.allclose((wph - wpl) / (2 * h), wp, atol=1e-6)
+    # smooth cut-off: everything vanishes at 12 A with zero slope
+    w12, wp12 = sg.shell_weights(np.array([12.0, 12.5]))
+    assert np.all(w12 == 0.0) and np.all(wp12 == 0.0)
+
+
+def small_system(seed, n=40):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-6, 6, (n, 3))
+    R = rng.choice([2.2, 2.05, 2.02, 2.3, 1.7], n)
+    heavy = R > 1.9
+    q = rng.normal(0, 0.3, n)
+    return pos, R, q, heavy
+
+
+def test_descriptors_match_brute_force():
+    pos, R, q, heavy = small_system(1)
+    F = sg.features(pos, R, q, heavy)
+    for i in range(len(R)):
+        f = np.zeros(64)
+        for j in range(len(R)):
+            if j == i or not heavy[j]:
+                continue
+            d = np.linalg.norm(pos[j] - pos[i])
+            if d >= 12.0:
+                continue
+            c = int(sg.channel(R[j]))
+            for k in range(16):
+                f[c * 16 + k] += sg.b3((d - sg.MU[k]) / sg.H) * sg.taper(d)
+        assert np.allclose(F[i, :64], f, atol=1e-12)
+    assert np.array_equal(F[:, 64], R) and np.array_equal(F[:, 65], q)
+
+
+def test_descriptor_gradient_matches_finite_differences():
+    pos, R, q, heavy = small_system(2)
+    G = np.random.default_rng(3).normal(size=(len(R), 64))
+    g = sg.feature_gradient(pos, R, heavy, G)
+    h = 1e-5
+    for a in (0, 7, 19):
+        for c in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[a, c] += h
+            p2[a, c] -= h
+            e1 = (G * sg.features(p1, R, q, heavy)[:, :64]).sum()
+            e2 = (G * sg.features(p2, R, q, heavy)[:, :64]).sum()
+            assert abs((e1 - e2) / (2 * h) - g[a, c]) < 1e-6 * max(1.0, abs(g[a, c]))
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    s = lambda *sh: rng.normal(0, 0.2, sh)
+    return sg.Params(s(66, 128), s(128), s(128, 128), s(128), s(128, 128), s(128), s(128), 0.1,
+                     rng.normal(0, 1, 66), rng.uniform(0.5, 2, 66), 5.0, 1.6)
+
+
+def test_mlp_input_gradient_matches_finite_differences():
+    P = random_params(4)
+    F = np.random.default_rng(5).normal(size=(6, 66))
+    s, cache = sg.forward(P, F, sg.exact)
+    dF = sg.input_gradient(P, cache, sg.exact)
+    h = 1e-6
+    for k in (0, 13, 40, 64, 65):
+        F1, F2 = F.copy(), F.copy()
+        F1[:, k] += h
+        F2[:, k] -= h
+        fd = (sg.forward(P, F1, sg.exact)[0] - sg.forward(P, F2, sg.exact)[0]) / (2 * h)
+        assert np.allclose(fd, dF[:, k], rtol=1e-5, atol=1e-8)
+
+
+def test_mlp_weight_gradients_match_finite_differences():
+    P = random_params(6)
+    rng = np.random.default_rng(7)
+    F = rng.normal(size=(9, 66))
+    t = rng.normal(5.0, 1.0, 9)
+    loss, g = sg.loss_and_weight_gradients(P, F, t, sg.exact)
+    h = 1e-6
+    for name, idx in (("W1", (3, 5)), ("b1", (7,)), ("W2", (10, 100)), ("W3", (127, 0)), ("b3", (64,)),
+                      ("w4", (12,))):
+        arr = getattr(P, name)
+        old = arr[idx]
+        arr[idx] = old + h
+        l1 = sg.loss_and_weight_gradients(P, F, t, sg.exact)[0]
+        arr[idx] = old - h
+        l2 = sg.loss_and_weight_gradients(P, F, t, sg.exact)[0]
+        arr[idx] = old
+        assert abs((l1 - l2) / (2 * h) - g[name][idx]) < 1e-6 * max(1.0, abs(g[name][idx])), name
