@@ -34,6 +34,10 @@ EXPORTED = (
     "agbnp3_batch_graph_capture", "agbnp3_batch_graph_launch", "agbnp3_batch_kernel_times",
     "agbnp3_batch_get_tuning", "agbnp3_batch_set_tuning", "agbnp3_batch_tune",
     "agbnp3_set_nblist_reuse", "agbnp3_batch_set_nblist_reuse", "agbnp3_nblist_rebuilds",
+    "agbnp3_sg_new", "agbnp3_sg_delete", "agbnp3_sg_size", "agbnp3_sg_get_params", "agbnp3_sg_set_params",
+    "agbnp3_sg_reserve", "agbnp3_batch_sg_collect", "agbnp3_sg_dataset", "agbnp3_sg_fit", "agbnp3_sg_train",
+    "agbnp3_sg_grads", "agbnp3_sg_infer", "agbnp3_sg_attach", "agbnp3_batch_sg_attach", "agbnp3_sg_stats",
+    "agbnp3_batch_sg_stats",
 )
 TUNING_PARAMS = ("rows", "desc_chunk", "chain_chunk", "desc_grid", "gb_grid", "chain_grid", "cap")
 NBLIST_EXPORTED = ("nblist_reset_neighbor_list", "nblist_delete_neighbor_list", "nblist_reallocate_neighbor_list")
@@ -115,6 +119,22 @@ def load():
         L.agbnp3_set_nblist_reuse.argtypes = [ci, ctypes.c_float, ci]
         L.agbnp3_batch_set_nblist_reuse.argtypes = [ci, ctypes.c_float, ci]
         L.agbnp3_nblist_rebuilds.argtypes = [ci, ctypes.POINTER(ci), ctypes.POINTER(ci)]
+        u64, f32 = ctypes.c_ulonglong, ctypes.c_float
+        L.agbnp3_sg_new.argtypes = [ctypes.POINTER(ci), u64]
+        L.agbnp3_sg_delete.argtypes = [ci]
+        L.agbnp3_sg_get_params.argtypes = [ci, vp, ci]
+        L.agbnp3_sg_set_params.argtypes = [ci, vp, ci]
+        L.agbnp3_sg_reserve.argtypes = [ci, ci]
+        L.agbnp3_batch_sg_collect.argtypes = [ci, ci, vp, vp, vp]
+        L.agbnp3_sg_dataset.argtypes = [ci, ctypes.POINTER(ci), vp, vp]
+        L.agbnp3_sg_fit.argtypes = [ci, f32, u64]
+        L.agbnp3_sg_train.argtypes = [ci, ci, ci, f32, f32, u64, vp]
+        L.agbnp3_sg_grads.argtypes = [ci, ci, ci, vp, vp]
+        L.agbnp3_sg_infer.argtypes = [ci, vp, ci, vp, vp]
+        L.agbnp3_sg_attach.argtypes = [ci, ci, ci]
+        L.agbnp3_batch_sg_attach.argtypes = [ci, ci, ci]
+        L.agbnp3_sg_stats.argtypes = [ci, vp, vp]
+        L.agbnp3_batch_sg_stats.argtypes = [ci, vp, vp]
         _nb.nblist_reallocate_neighbor_list.argtypes = [ctypes.POINTER(NeighList), ci, ci]
         _nb.nblist_reset_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
         _nb.nblist_delete_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
@@ -315,6 +335,16 @@ class Instance:
         if load().agbnp3_set_nblist_reuse(self.tag, float(skin), int(every)) != AGBNP_OK:
             raise RuntimeError("agbnp3_set_nblist_reuse failed")
 
+    def sg_attach(self, model, every):
+        if load().agbnp3_sg_attach(self.tag, model.id if model is not None else -1, int(every)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_attach failed")
+
+    def sg_stats(self):
+        err, mode = np.zeros(1), np.zeros(1, np.int32)
+        if load().agbnp3_sg_stats(self.tag, _p(err), _p(mode)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_stats failed")
+        return float(err[0]), int(mode[0])
+
     def nblist_rebuilds(self):
         """(calls, candidate rebuilds) since set_nblist_reuse."""
         c, r = ctypes.c_int(), ctypes.c_int()
@@ -397,6 +427,16 @@ class Batch:
         if load().agbnp3_batch_set_tuning(self.id, _p(p), p.size) != AGBNP_OK:
             raise RuntimeError("agbnp3_batch_set_tuning failed")
 
+    def sg_attach(self, model, every):
+        if load().agbnp3_batch_sg_attach(self.id, model.id if model is not None else -1, int(every)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_sg_attach failed")
+
+    def sg_stats(self):
+        err, mode = np.zeros(self.nsys), np.zeros(self.nsys, np.int32)
+        if load().agbnp3_batch_sg_stats(self.id, _p(err), _p(mode)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_sg_stats failed")
+        return err, mode
+
     def set_nblist_reuse(self, skin, every=0):
         if load().agbnp3_batch_set_nblist_reuse(self.id, float(skin), int(every)) != AGBNP_OK:
             raise RuntimeError("agbnp3_batch_set_nblist_reuse failed")
@@ -411,3 +451,82 @@ class Batch:
         if self.id is not None:
             load().agbnp3_batch_delete(self.id)
             self.id = None
+
+
+class Surrogate:
+    """A learned Born-radius model (agbnp3_sg_*)."""
+
+    NFEAT = 66
+    NPARAM = 41729      # W1 [66][128], b1, W2 [128][128], b2, W3, b3, w4 [128], b4
+    LAYOUT = (("W1", (66, 128)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)), ("W3", (128, 128)),
+              ("b3", (128,)), ("w4", (128,)), ("b4", (1,)))
+
+    def __init__(self, seed=0):
+        if agbnp3_initialize() != AGBNP_OK:
+            raise RuntimeError("agbnp3_initialize failed (no GPU?)")
+        m = ctypes.c_int(-1)
+        if load().agbnp3_sg_new(ctypes.byref(m), int(seed)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_new failed")
+        self.id = m.value
+
+    def delete(self):
+        if self.id is not None:
+            load().agbnp3_sg_delete(self.id)
+            self.id = None
+
+    def get_params(self):
+        n = load().agbnp3_sg_size()
+        out = np.zeros(n, np.float32)
+        if load().agbnp3_sg_get_params(self.id, _p(out), n) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_get_params failed")
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        if load().agbnp3_sg_set_params(self.id, _p(flat), flat.size) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_set_params failed")
+
+    def reserve(self, natoms):
+        if load().agbnp3_sg_reserve(self.id, int(natoms)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_reserve failed")
+
+    def collect(self, batch, x, y, z):
+        """Plain evaluation of `batch` at device coordinates (pointers), appended to the training set."""
+        if load().agbnp3_batch_sg_collect(batch.id, self.id, x, y, z) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_sg_collect failed")
+
+    def dataset(self):
+        c = ctypes.c_int(0)
+        load().agbnp3_sg_dataset(self.id, ctypes.byref(c), None, None)
+        F = np.zeros((c.value, self.NFEAT), np.float32)
+        t = np.zeros(c.value, np.float32)
+        if c.value and load().agbnp3_sg_dataset(self.id, None, _p(F), _p(t)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_dataset failed")
+        return F, t
+
+    def fit(self, margin=0.05, seed=0):
+        if load().agbnp3_sg_fit(self.id, float(margin), int(seed)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_fit failed")
+
+    def train(self, epochs, batch=4096, lr=2e-3, decay=0.9, seed=0):
+        out = np.zeros(epochs, np.float32)
+        if load().agbnp3_sg_train(self.id, int(epochs), int(batch), float(lr), float(decay), int(seed),
+                                  _p(out)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_train failed")
+        return out
+
+    def grads(self, r0, B):
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        if load().agbnp3_sg_grads(self.id, int(r0), int(B), _p(g), _p(loss)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_grads failed")
+        return float(loss[0]), g
+
+    def infer(self, F):
+        F = np.ascontiguousarray(F, np.float32)
+        n = F.shape[0]
+        s = np.zeros(n, np.float32)
+        d = np.zeros((n, 64), np.float32)
+        if load().agbnp3_sg_infer(self.id, _p(F), n, _p(s), _p(d)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_infer failed")
+        return s, d

@@ -16,7 +16,7 @@ import sys
 
 from . import _paths
 
-HIP_SOURCES = ["agbnp3_host.hip", "k_setup.hip", "k_tree.hip", "k_pairs.hip", "k_epi.hip", "k_hb.hip"]
+HIP_SOURCES = ["agbnp3_host.hip", "k_setup.hip", "k_tree.hip", "k_pairs.hip", "k_epi.hip", "k_hb.hip", "k_surrogate.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("AGBNP3_ARCH", "gfx950")
 HIP_FLAGS = [

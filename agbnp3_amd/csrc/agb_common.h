@@ -95,6 +95,37 @@ struct SysInfo {
   double df64;          // same factor in FP64
 };
 
+// ---- Born-radius surrogate (MLP 66 -> 128 -> 128 -> 128 -> 1)
+constexpr int kSgNShell = 16, kSgNFeat = 66, kSgKPad = 96, kSgHid = 128;
+// FP32 master parameter layout: W1 [66][128], b1, W2 [128][128], b2, W3, b3, w4 [128], b4
+constexpr int kSgOffW1 = 0;
+constexpr int kSgOffB1 = kSgOffW1 + kSgNFeat * kSgHid;
+constexpr int kSgOffW2 = kSgOffB1 + kSgHid;
+constexpr int kSgOffB2 = kSgOffW2 + kSgHid * kSgHid;
+constexpr int kSgOffW3 = kSgOffB2 + kSgHid;
+constexpr int kSgOffB3 = kSgOffW3 + kSgHid * kSgHid;
+constexpr int kSgOffW4 = kSgOffB3 + kSgHid;
+constexpr int kSgOffB4 = kSgOffW4 + kSgHid;
+constexpr int kSgNParam = kSgOffB4 + 1;
+
+// device view of a model: bf16 weights row-major [in][out] (backward) and
+// transposed [out][in] (forward, staged in LDS), FP32 biases / output layer
+// (pointers into the master parameters), normalisation and training window
+struct SgModel {
+  const unsigned short *w1, *w2, *w3, *wt1, *wt2, *wt3;
+  const float *b1, *b2, *b3, *w4, *b4;
+  const float *xmu, *xsd_inv, *xlo, *xhi;   // per input feature (66)
+  float ymu, ysd;
+};
+
+// training batch buffers (feature-major activations and deltas)
+struct SgTrain {
+  const unsigned short *X;   // normalised inputs [N][kSgKPad] bf16
+  const float *y;            // normalised targets [N]
+  unsigned short *A0, *A1, *A2, *D1, *D2, *D3;   // [features][B] bf16
+  float *A3, *dy, *err2;
+};
+
 // work-queue heads (cleared by the ingest kernel of every call)
 enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_COUNT };
 
@@ -170,6 +201,20 @@ struct Dev {
   WSite *ws;
   double *ws_part;                     // per-site energy khb*s
   double *epart;                       // [EP_COUNT][nablk]
+  // Born-radius surrogate (sg_on): per system [0] calls, [1] anchored,
+  // [2] mode of this call (1: surrogate), [3] descriptors out of range
+  int sg_on, sg_every, sg_grid;
+  SgModel sg;
+  const int *sys_sg;                   // per system: model attached
+  int *sg_state;
+  const int2 *sg_rows;                 // 128-atom row blocks inside systems
+  int nsg_rows;
+  float *sg_F;                         // raw shell descriptors [n][64]
+  unsigned short *sg_X;                // normalised bf16 input [n][kSgKPad]
+  float *sg_s, *sg_dsdF;               // predicted s [n], ds/dF [n][64]
+  float *sg_target;                    // plain descreening sum of the call [n]
+  double *sg_c, *sg_delta;             // anchor corrections, surrogate error at the anchor [n]
+  double *sg_err;                      // per system: first-order Delta G error estimate
 };
 
 // launch shapes chosen per pack (the autotuner may change them)
@@ -197,6 +242,19 @@ void launch_uv(const Dev &d, hipStream_t s);
 hipError_t prepare_pairs(int ntypes);
 void launch_ws_place(const Dev &d, hipStream_t s);
 void launch_ws_free(const Dev &d, hipStream_t s);
+void launch_sg_front(const Dev &d, hipStream_t s);
+void launch_sg_grad(const Dev &d, hipStream_t s);
+void launch_sg_judge(const Dev &d, hipStream_t s);
+hipError_t prepare_surrogate();
+void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t s);
+void launch_sg_infer(const SgModel &m, const unsigned short *X, int n, float *s_out, float *dsdF, int grid,
+                     hipStream_t s);
+size_t sg_mlp_lds();
+void launch_sg_train_step(const SgModel &m, const SgTrain &t, int r0, int B, float *grad, float *loss,
+                          int grid, hipStream_t s);
+void launch_sg_adam(float *p, const float *g, float *mom, float *vel, float lr, int step, hipStream_t s);
+void launch_sg_pack(const float *p, unsigned short *w1, unsigned short *w2, unsigned short *w3, unsigned short *wt1,
+                    unsigned short *wt2, unsigned short *wt3, hipStream_t s);
 void launch_final(const Dev &d, double *sp, double *br, double *sa, double *dgbdr, double *dvwdr,
                   double *decav, double *dehb, double *energies, hipStream_t s);
 

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <random>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -188,7 +189,51 @@ struct Pack {
   // captured pipeline
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // surrogate buffers (allocated on first attach)
+  std::vector<void *> sg_allocs;
+  int sg_model = -1;
 };
+
+// ------------------------------------------------------------ surrogate
+// A trained (or training) Born-radius model: FP32 master parameters, Adam
+// state, the bf16 copies the kernels read, the normalisation and the
+// training window, and a training set on the device.
+struct SgHost {
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
+  unsigned short *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *wt1 = nullptr, *wt2 = nullptr, *wt3 = nullptr;
+  float *norm = nullptr;          // xmu, xsd_inv, xlo, xhi [4][kSgNFeat]
+  float ymu = 0.f, ysd = 1.f;
+  int step = 0;
+  // training set
+  int cap = 0, count = 0;
+  float *F = nullptr, *target = nullptr;          // raw [cap][66], [cap]
+  unsigned short *X = nullptr;                    // normalised bf16 [cap][96]
+  float *y = nullptr;                             // normalised targets [cap]
+  int bcap = 0;
+  SgTrain tb{};
+  std::vector<void *> allocs, tallocs;
+  hipStream_t stream = nullptr;
+  SgModel view() const {
+    SgModel v{};
+    v.w1 = w1; v.w2 = w2; v.w3 = w3; v.wt1 = wt1; v.wt2 = wt2; v.wt3 = wt3;
+    v.b1 = p + kSgOffB1; v.b2 = p + kSgOffB2; v.b3 = p + kSgOffB3; v.w4 = p + kSgOffW4; v.b4 = p + kSgOffB4;
+    v.xmu = norm; v.xsd_inv = norm + kSgNFeat; v.xlo = norm + 2 * kSgNFeat; v.xhi = norm + 3 * kSgNFeat;
+    v.ymu = ymu;
+    v.ysd = ysd;
+    return v;
+  }
+};
+std::vector<SgHost *> g_models;
+
+void sg_free(SgHost *M) {
+  if (M->stream) (void)hipStreamSynchronize(M->stream);
+  for (void *q : M->allocs) (void)hipFree(q);
+  for (void *q : M->tallocs) (void)hipFree(q);
+  if (M->F) (void)hipFree(M->F);
+  if (M->target) (void)hipFree(M->target);
+  if (M->stream) (void)hipStreamDestroy(M->stream);
+  delete M;
+}
 
 struct Instance {
   Model m;
@@ -269,8 +314,10 @@ void free_pack(Pack &P) {
   drop_graph(P);
   for (void *p : P.allocs) (void)hipFree(p);
   for (void *p : P.item_allocs) (void)hipFree(p);
+  for (void *p : P.sg_allocs) (void)hipFree(p);
   P.allocs.clear();
   P.item_allocs.clear();
+  P.sg_allocs.clear();
   if (P.hstage) (void)hipHostFree(P.hstage);
   P.hstage = nullptr;
   for (auto &set : P.ev_sets)
@@ -614,6 +661,8 @@ int run_pipeline(Pack &P, const double *x, const double *y, const double *z, dou
   mark("tree_volumes");
   launch_scale(d, s);
   mark("scale");
+  launch_sg_front(d, s);
+  mark("sg_front");
   launch_descreen(d, t, s);
   mark("descreen");
   launch_born(d, s);
@@ -622,8 +671,11 @@ int run_pipeline(Pack &P, const double *x, const double *y, const double *z, dou
   mark("gbpair");
   launch_q2ab(d, s);
   mark("q2ab");
+  launch_sg_judge(d, s);
   launch_chain(d, t, s);
   mark("chain");
+  launch_sg_grad(d, s);
+  mark("sg_grad");
   launch_ws_place(d, s);
   mark("ws_place");
   launch_ws_free(d, s);
@@ -960,6 +1012,11 @@ void agbnp3_terminate(void) {
     b = nullptr;
   }
   g_batches.clear();
+  for (auto &m : g_models) {
+    if (m) sg_free(m);
+    m = nullptr;
+  }
+  g_models.clear();
   g_initialized = false;
 }
 
@@ -1494,6 +1551,439 @@ int agbnp3_batch_tune(int batch, const double *x, const double *y, const double 
   Batch *B = get_batch(batch, "agbnp3_batch_tune()");
   if (!B || !batch_topology_ok(B, "agbnp3_batch_tune()")) return AGBNP_ERR;
   return pack_tune(B->p, x, y, z, reps, report, report_len);
+}
+
+// ---------------------------------------------------------------- surrogate
+static SgHost *get_model(int id, const char *who) {
+  if (id < 0 || id >= (int)g_models.size() || !g_models[id]) {
+    errprint("%s: invalid surrogate model %d.\n", who, id);
+    return nullptr;
+  }
+  return g_models[id];
+}
+
+
+static unsigned short host_bf16(float f) {
+  unsigned int u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);   // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+static int sg_repack(SgHost *M) {
+  launch_sg_pack(M->p, M->w1, M->w2, M->w3, M->wt1, M->wt2, M->wt3, M->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(M->stream));
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_new(int *model, unsigned long long seed) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  if (!g_initialized) {
+    errprint("agbnp3_sg_new(): agbnp library is not initialized.\n");
+    return AGBNP_ERR;
+  }
+  if (prepare_surrogate() != hipSuccess) {
+    errprint("agbnp3_sg_new(): the MLP kernels need %zu bytes of LDS per workgroup.\n", sg_mlp_lds());
+    return AGBNP_ERR;
+  }
+  SgHost *M = new SgHost();
+  auto &L = M->allocs;
+  if (dalloc(L, &M->p, kSgNParam) || dalloc(L, &M->g, kSgNParam) || dalloc(L, &M->m, kSgNParam) ||
+      dalloc(L, &M->v, kSgNParam) || dalloc(L, &M->loss, 1 << 16) || dalloc(L, &M->w1, kSgKPad * kSgHid) ||
+      dalloc(L, &M->wt1, kSgKPad * kSgHid) || dalloc(L, &M->w2, kSgHid * kSgHid) ||
+      dalloc(L, &M->wt2, kSgHid * kSgHid) || dalloc(L, &M->w3, kSgHid * kSgHid) ||
+      dalloc(L, &M->wt3, kSgHid * kSgHid) || dalloc(L, &M->norm, 4 * kSgNFeat)) {
+    sg_free(M);
+    return AGBNP_ERR;
+  }
+  if (hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking) != hipSuccess) {
+    sg_free(M);
+    return AGBNP_ERR;
+  }
+  // Glorot-uniform weights, zero biases; identity normalisation, open window
+  std::mt19937_64 rng(seed);
+  std::vector<float> p(kSgNParam, 0.f);
+  auto fill = [&](int off, int fin, int fout) {
+    const float a = sqrtf(6.0f / (float)(fin + fout));
+    std::uniform_real_distribution<float> u(-a, a);
+    for (int k = 0; k < fin * fout; k++) p[off + k] = u(rng);
+  };
+  fill(kSgOffW1, kSgNFeat, kSgHid);
+  fill(kSgOffW2, kSgHid, kSgHid);
+  fill(kSgOffW3, kSgHid, kSgHid);
+  fill(kSgOffW4, kSgHid, 1);
+  std::vector<float> norm(4 * kSgNFeat);
+  for (int k = 0; k < kSgNFeat; k++) {
+    norm[k] = 0.f;
+    norm[kSgNFeat + k] = 1.f;
+    norm[2 * kSgNFeat + k] = -INFINITY;
+    norm[3 * kSgNFeat + k] = INFINITY;
+  }
+  if (hipMemcpy(M->p, p.data(), sizeof(float) * kSgNParam, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(M->norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(M->m, 0, sizeof(float) * kSgNParam) != hipSuccess ||
+      hipMemset(M->v, 0, sizeof(float) * kSgNParam) != hipSuccess || sg_repack(M) != AGBNP_OK) {
+    sg_free(M);
+    return AGBNP_ERR;
+  }
+  int slot = 0;
+  while (slot < (int)g_models.size() && g_models[slot]) slot++;
+  if (slot == (int)g_models.size()) g_models.push_back(nullptr);
+  g_models[slot] = M;
+  *model = slot;
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_delete(int model) {
+  std::lock_guard<std::mutex> lk(g_mutex);
+  SgHost *M = get_model(model, "agbnp3_sg_delete()");
+  if (!M) return AGBNP_ERR;
+  for (Instance *I : g_slots)
+    if (I && I->p.sg_model == model) { I->p.d.sg_on = 0; I->p.sg_model = -1; drop_graph(I->p); }
+  for (Batch *B : g_batches)
+    if (B && B->p.sg_model == model) { B->p.d.sg_on = 0; B->p.sg_model = -1; drop_graph(B->p); }
+  sg_free(M);
+  g_models[model] = nullptr;
+  return AGBNP_OK;
+}
+
+// export / import: params (kSgNParam), xmu, xsd_inv, xlo, xhi (kSgNFeat each), ymu, ysd
+int agbnp3_sg_size(void) { return kSgNParam + 4 * kSgNFeat + 2; }
+
+int agbnp3_sg_get_params(int model, float *out, int n) {
+  SgHost *M = get_model(model, "agbnp3_sg_get_params()");
+  if (!M || n < agbnp3_sg_size()) return AGBNP_ERR;
+  HIPCHK(hipStreamSynchronize(M->stream));
+  HIPCHK(hipMemcpy(out, M->p, sizeof(float) * kSgNParam, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out + kSgNParam, M->norm, sizeof(float) * 4 * kSgNFeat, hipMemcpyDeviceToHost));
+  out[kSgNParam + 4 * kSgNFeat] = M->ymu;
+  out[kSgNParam + 4 * kSgNFeat + 1] = M->ysd;
+  return AGBNP_OK;
+}
+
+static void sg_refresh_attached(int model) {
+  SgHost *M = g_models[model];
+  for (Instance *I : g_slots)
+    if (I && I->p.sg_model == model) { I->p.d.sg = M->view(); drop_graph(I->p); }
+  for (Batch *B : g_batches)
+    if (B && B->p.sg_model == model) { B->p.d.sg = M->view(); drop_graph(B->p); }
+}
+
+int agbnp3_sg_set_params(int model, const float *in, int n) {
+  SgHost *M = get_model(model, "agbnp3_sg_set_params()");
+  if (!M || n < agbnp3_sg_size()) return AGBNP_ERR;
+  HIPCHK(hipStreamSynchronize(M->stream));
+  HIPCHK(hipMemcpy(M->p, in, sizeof(float) * kSgNParam, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(M->norm, in + kSgNParam, sizeof(float) * 4 * kSgNFeat, hipMemcpyHostToDevice));
+  M->ymu = in[kSgNParam + 4 * kSgNFeat];
+  M->ysd = in[kSgNParam + 4 * kSgNFeat + 1];
+  if (sg_repack(M) != AGBNP_OK) return AGBNP_ERR;
+  sg_refresh_attached(model);
+  return AGBNP_OK;
+}
+
+// surrogate on a pack: every = anchor period (plain call every `every`
+// calls, 1 = always plain), 0 detaches
+static int pack_sg_attach(Pack &P, int model, int every) {
+  HIPCHK(hipStreamSynchronize(P.stream));
+  Dev &d = P.d;
+  drop_graph(P);
+  if (every <= 0) {
+    d.sg_on = 0;
+    P.sg_model = -1;
+    return AGBNP_OK;
+  }
+  SgHost *M = get_model(model, "agbnp3_sg_attach()");
+  if (!M) return AGBNP_ERR;
+  HIPCHK(prepare_surrogate());
+  if (P.sg_allocs.empty()) {
+    const size_t nn = std::max(P.n, 1);
+    std::vector<int> flags(P.nsys, 1);
+    std::vector<int2> rows;
+    for (const SysInfo &S : P.sinfo)
+      for (int a = S.a0; a < S.a0 + S.n; a += 128) rows.push_back(make_int2(a, std::min(a + 128, S.a0 + S.n)));
+    auto &L = P.sg_allocs;
+    if (upload(L, &d.sys_sg, flags) || upload(L, &d.sg_rows, rows) || dalloc(L, &d.sg_state, 4 * (size_t)P.nsys) ||
+        dalloc(L, &d.sg_F, 64 * nn) || dalloc(L, &d.sg_X, (size_t)kSgKPad * nn) || dalloc(L, &d.sg_s, nn) ||
+        dalloc(L, &d.sg_dsdF, 64 * nn) || dalloc(L, &d.sg_target, nn) || dalloc(L, &d.sg_c, nn) ||
+        dalloc(L, &d.sg_delta, nn) || dalloc(L, &d.sg_err, (size_t)P.nsys))
+      return AGBNP_ERR;
+    d.nsg_rows = (int)rows.size();
+  }
+  HIPCHK(hipMemset(d.sg_state, 0, sizeof(int) * 4 * P.nsys));
+  HIPCHK(hipMemset(d.sg_c, 0, sizeof(double) * std::max(P.n, 1)));
+  HIPCHK(hipMemset(d.sg_err, 0, sizeof(double) * P.nsys));
+  int dev = 0, cu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) cu = prop.multiProcessorCount;
+  d.sg_on = 1;
+  d.sg_every = every;
+  d.sg_grid = cu;
+  d.sg = M->view();
+  P.sg_model = model;
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_attach(int tag, int model, int every) {
+  Instance *I = get_instance(tag, "agbnp3_sg_attach()");
+  if (!I) return AGBNP_ERR;
+  return pack_sg_attach(I->p, model, every);
+}
+
+int agbnp3_batch_sg_attach(int batch, int model, int every) {
+  Batch *B = get_batch(batch, "agbnp3_batch_sg_attach()");
+  if (!B) return AGBNP_ERR;
+  return pack_sg_attach(B->p, model, every);
+}
+
+static int pack_sg_stats(Pack &P, double *err, int *mode) {
+  if (!P.d.sg_on) {
+    errprint("agbnp3_sg_stats(): no surrogate attached.\n");
+    return AGBNP_ERR;
+  }
+  HIPCHK(hipStreamSynchronize(P.stream));
+  if (err) HIPCHK(hipMemcpy(err, P.d.sg_err, sizeof(double) * P.nsys, hipMemcpyDeviceToHost));
+  if (mode) {
+    std::vector<int> st(4 * P.nsys);
+    HIPCHK(hipMemcpy(st.data(), P.d.sg_state, sizeof(int) * st.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < P.nsys; b++) mode[b] = st[4 * b + 2];
+  }
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_stats(int tag, double *err, int *mode) {
+  Instance *I = get_instance(tag, "agbnp3_sg_stats()");
+  if (!I) return AGBNP_ERR;
+  return pack_sg_stats(I->p, err, mode);
+}
+
+int agbnp3_batch_sg_stats(int batch, double *err, int *mode) {
+  Batch *B = get_batch(batch, "agbnp3_batch_sg_stats()");
+  if (!B) return AGBNP_ERR;
+  return pack_sg_stats(B->p, err, mode);
+}
+
+// training set: plain evaluations of a batch at device coordinates x, y, z
+int agbnp3_sg_reserve(int model, int natoms) {
+  SgHost *M = get_model(model, "agbnp3_sg_reserve()");
+  if (!M || natoms <= 0) return AGBNP_ERR;
+  if (natoms <= M->cap) return AGBNP_OK;
+  HIPCHK(hipStreamSynchronize(M->stream));
+  float *F = nullptr, *t = nullptr;
+  if (hipMalloc(&F, sizeof(float) * (size_t)natoms * kSgNFeat) != hipSuccess ||
+      hipMalloc(&t, sizeof(float) * (size_t)natoms) != hipSuccess) {
+    if (F) (void)hipFree(F);
+    errprint("agbnp3_sg_reserve(): cannot allocate a training set of %d atoms.\n", natoms);
+    return AGBNP_ERR;
+  }
+  if (M->count) {
+    HIPCHK(hipMemcpy(F, M->F, sizeof(float) * (size_t)M->count * kSgNFeat, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(t, M->target, sizeof(float) * (size_t)M->count, hipMemcpyDeviceToDevice));
+  }
+  if (M->F) { (void)hipFree(M->F); (void)hipFree(M->target); }
+  M->F = F;
+  M->target = t;
+  M->cap = natoms;
+  return AGBNP_OK;
+}
+
+int agbnp3_batch_sg_collect(int batch, int model, const double *x, const double *y, const double *z) {
+  Batch *B = get_batch(batch, "agbnp3_batch_sg_collect()");
+  SgHost *M = get_model(model, "agbnp3_batch_sg_collect()");
+  if (!B || !M || !batch_topology_ok(B, "agbnp3_batch_sg_collect()")) return AGBNP_ERR;
+  Pack &P = B->p;
+  if (M->count + P.n > M->cap && agbnp3_sg_reserve(model, std::max(2 * M->cap, M->count + P.n)) != AGBNP_OK)
+    return AGBNP_ERR;
+  const int keep_model = P.sg_model, keep_every = P.d.sg_every, keep_on = P.d.sg_on;
+  if (pack_sg_attach(P, model, 1) != AGBNP_OK) return AGBNP_ERR;      // plain calls with descriptors
+  const size_t n = P.n;
+  double *o = P.dout;
+  int rc = run_pipeline(P, x, y, z, o, o + n, o + 2 * n, o + 3 * n, o + 6 * n, o + 9 * n, o + 12 * n, o + 15 * n,
+                        false);
+  if (rc == AGBNP_OK) rc = pack_sync(P, "agbnp3_batch_sg_collect()");
+  if (rc == AGBNP_OK) {
+    launch_sg_append(P.d, M->F, M->target, M->count, P.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(P.stream));
+    M->count += P.n;
+  }
+  if (keep_on) pack_sg_attach(P, keep_model, keep_every);
+  else pack_sg_attach(P, -1, 0);
+  return rc;
+}
+
+int agbnp3_sg_dataset(int model, int *count, float *F, float *target) {
+  SgHost *M = get_model(model, "agbnp3_sg_dataset()");
+  if (!M) return AGBNP_ERR;
+  if (count) *count = M->count;
+  HIPCHK(hipStreamSynchronize(M->stream));
+  if (F && M->count) HIPCHK(hipMemcpy(F, M->F, sizeof(float) * (size_t)M->count * kSgNFeat, hipMemcpyDeviceToHost));
+  if (target && M->count) HIPCHK(hipMemcpy(target, M->target, sizeof(float) * M->count, hipMemcpyDeviceToHost));
+  return AGBNP_OK;
+}
+
+// normalisation (mean / standard deviation per feature and of the target),
+// the training window (observed range widened by `margin` of its width on
+// each side), and the shuffled, normalised bf16 training matrix
+int agbnp3_sg_fit(int model, float margin, unsigned long long seed) {
+  SgHost *M = get_model(model, "agbnp3_sg_fit()");
+  if (!M || M->count < 2) {
+    errprint("agbnp3_sg_fit(): no training set.\n");
+    return AGBNP_ERR;
+  }
+  const int N = M->count;
+  std::vector<float> F((size_t)N * kSgNFeat), t(N);
+  if (agbnp3_sg_dataset(model, nullptr, F.data(), t.data()) != AGBNP_OK) return AGBNP_ERR;
+  std::vector<double> mu(kSgNFeat, 0.0), sd(kSgNFeat, 0.0);
+  double ym = 0.0, ys = 0.0;
+  for (int i = 0; i < N; i++) {
+    for (int k = 0; k < kSgNFeat; k++) mu[k] += F[(size_t)i * kSgNFeat + k];
+    ym += t[i];
+  }
+  for (int k = 0; k < kSgNFeat; k++) mu[k] /= N;
+  ym /= N;
+  for (int i = 0; i < N; i++) {
+    for (int k = 0; k < kSgNFeat; k++) {
+      const double e = F[(size_t)i * kSgNFeat + k] - mu[k];
+      sd[k] += e * e;
+    }
+    ys += (t[i] - ym) * (t[i] - ym);
+  }
+  std::vector<float> norm(4 * kSgNFeat);
+  for (int k = 0; k < kSgNFeat; k++) {
+    const double s = std::max(sqrt(sd[k] / N), 1e-3);
+    norm[k] = (float)mu[k];
+    norm[kSgNFeat + k] = (float)(1.0 / s);
+  }
+  std::vector<float> lo(kSgNFeat, INFINITY), hi(kSgNFeat, -INFINITY);
+  for (int i = 0; i < N; i++)
+    for (int k = 0; k < kSgNFeat; k++) {
+      const float v = (F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k];
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
+    }
+  for (int k = 0; k < kSgNFeat; k++) {
+    const float w = std::max(hi[k] - lo[k], 1e-3f);
+    norm[2 * kSgNFeat + k] = lo[k] - margin * w;
+    norm[3 * kSgNFeat + k] = hi[k] + margin * w;
+  }
+  M->ymu = (float)ym;
+  M->ysd = (float)std::max(sqrt(ys / N), 1e-6);
+  std::vector<int> perm(N);
+  for (int i = 0; i < N; i++) perm[i] = i;
+  std::mt19937_64 rng(seed);
+  std::shuffle(perm.begin(), perm.end(), rng);
+  std::vector<unsigned short> X((size_t)N * kSgKPad, 0);
+  std::vector<float> y(N);
+  for (int r = 0; r < N; r++) {
+    const int i = perm[r];
+    for (int k = 0; k < kSgNFeat; k++)
+      X[(size_t)r * kSgKPad + k] = host_bf16((F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k]);
+    y[r] = (float)((t[i] - ym) / M->ysd);
+  }
+  HIPCHK(hipStreamSynchronize(M->stream));
+  for (void *q : M->tallocs) (void)hipFree(q);
+  M->tallocs.clear();
+  M->bcap = 0;
+  if (upload(M->tallocs, (const unsigned short **)&M->X, X) || upload(M->tallocs, (const float **)&M->y, y))
+    return AGBNP_ERR;
+  HIPCHK(hipMemcpy(M->norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice));
+  sg_refresh_attached(model);
+  return AGBNP_OK;
+}
+
+static int sg_batch_buffers(SgHost *M, int B) {
+  if (B <= M->bcap) return AGBNP_OK;
+  auto &L = M->tallocs;
+  SgTrain &t = M->tb;
+  if (dalloc(L, &t.A0, (size_t)kSgKPad * B) || dalloc(L, &t.A1, (size_t)kSgHid * B) ||
+      dalloc(L, &t.A2, (size_t)kSgHid * B) || dalloc(L, &t.D1, (size_t)kSgHid * B) ||
+      dalloc(L, &t.D2, (size_t)kSgHid * B) || dalloc(L, &t.D3, (size_t)kSgHid * B) ||
+      dalloc(L, &t.A3, (size_t)kSgHid * B) || dalloc(L, &t.dy, (size_t)B) || dalloc(L, &t.err2, (size_t)B))
+    return AGBNP_ERR;
+  M->bcap = B;
+  return AGBNP_OK;
+}
+
+// gradients of one batch of the shuffled training set (no update): tests
+int agbnp3_sg_grads(int model, int r0, int B, float *grad, float *loss) {
+  SgHost *M = get_model(model, "agbnp3_sg_grads()");
+  if (!M || !M->X || B <= 0 || r0 < 0 || r0 + B > M->count) return AGBNP_ERR;
+  if (sg_batch_buffers(M, B) != AGBNP_OK) return AGBNP_ERR;
+  M->tb.X = M->X;
+  M->tb.y = M->y;
+  launch_sg_train_step(M->view(), M->tb, r0, B, M->g, M->loss, 256, M->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(M->stream));
+  if (grad) HIPCHK(hipMemcpy(grad, M->g, sizeof(float) * kSgNParam, hipMemcpyDeviceToHost));
+  if (loss) HIPCHK(hipMemcpy(loss, M->loss, sizeof(float), hipMemcpyDeviceToHost));
+  return AGBNP_OK;
+}
+
+// Adam over `epochs` passes of mini-batches of B rows in a seeded random
+// order; lr_e = lr decay^e; epoch_loss[e] = mean batch loss
+int agbnp3_sg_train(int model, int epochs, int B, float lr, float decay, unsigned long long seed, float *epoch_loss) {
+  SgHost *M = get_model(model, "agbnp3_sg_train()");
+  if (!M || !M->X || B <= 0 || epochs <= 0) return AGBNP_ERR;
+  const int nb = M->count / B;
+  if (nb <= 0 || nb > (1 << 16)) {
+    errprint("agbnp3_sg_train(): %d rows do not make batches of %d.\n", M->count, B);
+    return AGBNP_ERR;
+  }
+  if (sg_batch_buffers(M, B) != AGBNP_OK) return AGBNP_ERR;
+  M->tb.X = M->X;
+  M->tb.y = M->y;
+  int dev = 0, cu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) cu = prop.multiProcessorCount;
+  std::mt19937_64 rng(seed);
+  std::vector<int> order(nb);
+  std::vector<float> losses(nb);
+  for (int e = 0; e < epochs; e++) {
+    for (int b = 0; b < nb; b++) order[b] = b;
+    std::shuffle(order.begin(), order.end(), rng);
+    const float lre = lr * powf(decay, (float)e);
+    for (int b = 0; b < nb; b++) {
+      launch_sg_train_step(M->view(), M->tb, order[b] * B, B, M->g, M->loss + b, cu, M->stream);
+      launch_sg_adam(M->p, M->g, M->m, M->v, lre, ++M->step, M->stream);
+      launch_sg_pack(M->p, M->w1, M->w2, M->w3, M->wt1, M->wt2, M->wt3, M->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(losses.data(), M->loss, sizeof(float) * nb, hipMemcpyDeviceToHost, M->stream));
+    HIPCHK(hipStreamSynchronize(M->stream));
+    double s = 0.0;
+    for (float l : losses) s += l;
+    if (epoch_loss) epoch_loss[e] = (float)(s / nb);
+  }
+  return AGBNP_OK;
+}
+
+// the MLP on raw descriptors F [n][66] (host): s [n], ds/dF [n][64]
+int agbnp3_sg_infer(int model, const float *F, int n, float *s_out, float *dsdF) {
+  SgHost *M = get_model(model, "agbnp3_sg_infer()");
+  if (!M || n <= 0) return AGBNP_ERR;
+  std::vector<float> norm(4 * kSgNFeat);
+  HIPCHK(hipStreamSynchronize(M->stream));
+  HIPCHK(hipMemcpy(norm.data(), M->norm, sizeof(float) * norm.size(), hipMemcpyDeviceToHost));
+  std::vector<unsigned short> X((size_t)n * kSgKPad, 0);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < kSgNFeat; k++)
+      X[(size_t)i * kSgKPad + k] = host_bf16((F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k]);
+  std::vector<void *> tmp;
+  const unsigned short *dX = nullptr;
+  float *ds = nullptr, *dd = nullptr;
+  int rc = AGBNP_ERR;
+  if (!upload(tmp, &dX, X) && !dalloc(tmp, &ds, (size_t)n) && !dalloc(tmp, &dd, (size_t)n * 64)) {
+    launch_sg_infer(M->view(), dX, n, ds, dd, 256, M->stream);
+    if (hipGetLastError() == hipSuccess && hipStreamSynchronize(M->stream) == hipSuccess &&
+        hipMemcpy(s_out, ds, sizeof(float) * n, hipMemcpyDeviceToHost) == hipSuccess &&
+        (!dsdF || hipMemcpy(dsdF, dd, sizeof(float) * (size_t)n * 64, hipMemcpyDeviceToHost) == hipSuccess))
+      rc = AGBNP_OK;
+  }
+  for (void *q : tmp) (void)hipFree(q);
+  return rc;
 }
 
 }  // extern "C"

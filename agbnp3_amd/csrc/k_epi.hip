@@ -85,7 +85,20 @@ __global__ __launch_bounds__(kEpi) void k_born(Dev d) {
   const int i = blk.x + threadIdx.x;
   double evdw = 0.0, ecorr = 0.0, eself = 0.0;
   if (i < blk.y) {
-    const double s = fix_get(d.acc_beta[i]);
+    double s = fix_get(d.acc_beta[i]);
+    if (d.sg_on && d.sys_sg[d.asys[i]]) {
+      // surrogate call: the model's sum plus the correction of the last
+      // anchor; anchor call: record the surrogate's error and re-anchor
+      const int mode = d.sg_state[4 * d.asys[i] + 2];
+      const double pred = (double)d.sg_s[i] + d.sg_c[i];
+      if (mode == 1) {
+        s = pred;
+      } else {
+        d.sg_delta[i] = pred - s;
+        d.sg_c[i] = s - (double)d.sg_s[i];
+        d.sg_target[i] = (float)s;
+      }
+    }
     const double beta = 1.0 / ((double)d.R[i] - kRadInc) - s / (4.0 * kPi);
     double bf, sw;
     if (beta < 0.0) {

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
   int staged = -1;
   for (int it = next_item(&d.queue[Q_DESC], &slot); it < d.ndesc_items; it = next_item(&d.queue[Q_DESC], &slot)) {
     const int4 item = d.desc_items[it];
+    if (d.sg_on && d.sg_state[4 * d.rblk[item.x].y + 2] == 1) continue;   // surrogate call for this system
     const int t = d.rblk[item.x].x;
     if (t != staged) {                 // uniform: every thread reads the same item
       stage_tables<ROWS>(d, dyn_tab, t, true);
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(ROWS) void k_chain(Dev d) {
        it = next_item(&d.queue[Q_CHAIN], &slot)) {
     const int4 item = d.chain_items[it];
     const int4 rb = d.rblk[item.x];
+    if (d.sg_on && d.sg_state[4 * rb.y + 2] == 1) continue;   // surrogate call: k_sg_grad instead
     const int t = rb.x;
     if (t != staged) {
       stage_tables<ROWS>(d, dyn_tab, t, true);

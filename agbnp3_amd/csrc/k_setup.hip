@@ -18,6 +18,7 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
                                                 const double *__restrict__ y,
                                                 const double *__restrict__ z) {
   const int i = blockIdx.x * 256 + threadIdx.x;
+  if (d.sg_on && i < d.nsys) d.sg_state[4 * i + 3] = 0;   // surrogate: out-of-range flag of the call
   if (i < d.n) {
     const int e = d.int2ext[i];
     d.pos[i] = make_float4((float)x[e], (float)y[e], (float)z[e], d.R[i]);

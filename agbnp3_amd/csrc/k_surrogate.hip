@@ -1,0 +1,661 @@
+// Learned Born-radius surrogate: a per-atom MLP on local-environment
+// descriptors predicts the descreening sum s_i of agbnp3's inverse Born
+// radius (beta_i = 1/(R_i - 0.5) - s_i / 4pi), with its gradient, and the
+// kernels that train it.  The surrogate has no counterpart in the reference;
+// it stands in for the descreening and chain-rule pair kernels on the calls
+// between plain anchor calls.
+//
+// Descriptors (per atom i): four heavy-atom channels (by R) x 16 radial
+// shells up to 12 A, F[c][k] = sum_j B3((d_ij - mu_k)/h) T(d_ij) with
+// h = 0.75 A, mu_k = h (k + 1/2), B3 the cubic B-spline and T a cosine taper
+// on [10.5, 12] A; then R_i and q_i.  Normalised and rounded to bf16 they are
+// the MLP input, padded to 96 columns.
+//
+// MLP: 66 -> 128 -> 128 -> 128 -> 1, SiLU, bf16 operands on the matrix cores
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation and biases.  One block
+// (4 waves) holds the three weight matrices in LDS and runs 32-atom tiles:
+// forward, then the input gradient ds/dF back through the layers (the
+// transposed products read the row-major weights from L2).
+//
+// MFMA 16x16x32 lane layout: lane l holds A[l%16][8(l/16)+r],
+// B[8(l/16)+r][l%16] (r = 0..7) and D[4(l/16)+r][l%16] (r = 0..3).
+#include "agb_common.h"
+#include "agb_device.h"
+
+namespace agb {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float kSgH = 0.75f, kSgCut = 12.0f, kSgTaper0 = 10.5f;
+constexpr int kSgTile = 32;                 // atoms per MLP tile
+constexpr int kLd1 = kSgKPad + 8;           // LDS row strides (bf16 elements), padded
+constexpr int kLdH = kSgHid + 8;
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silup(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// cubic B-spline and its derivative
+__device__ __forceinline__ float bsp3(float t) {
+  const float a = fabsf(t);
+  if (a < 1.f) return 2.f / 3.f - a * a + 0.5f * a * a * a;
+  if (a < 2.f) { const float u = 2.f - a; return u * u * u * (1.f / 6.f); }
+  return 0.f;
+}
+__device__ __forceinline__ float bsp3p(float t) {
+  const float a = fabsf(t);
+  float v = 0.f;
+  if (a < 1.f) v = -2.f * a + 1.5f * a * a;
+  else if (a < 2.f) { const float u = 2.f - a; v = -0.5f * u * u; }
+  return t < 0.f ? -v : v;
+}
+__device__ __forceinline__ void sg_taper(float d, float &T, float &Tp) {
+  if (d <= kSgTaper0) { T = 1.f; Tp = 0.f; return; }
+  if (d >= kSgCut) { T = 0.f; Tp = 0.f; return; }
+  const float w = kSgCut - kSgTaper0, x = (d - kSgTaper0) / w;
+  T = 0.5f * (1.f + cospif(x));
+  Tp = -0.5f * kPiF / w * sinpif(x);
+}
+__device__ __forceinline__ int sg_channel(float R) {
+  return R < 2.035f ? 0 : (R < 2.125f ? 1 : (R < 2.25f ? 2 : 3));
+}
+
+// ------------------------------------------------------------ descriptors
+// item = a block of up to kSgRows consecutive atoms of one system against
+// that system's heavy atoms; each thread owns one row's 64 shell sums in LDS
+constexpr int kSgRows = 128;
+
+__global__ __launch_bounds__(kSgRows) void k_sg_features(Dev d) {
+  __shared__ float F[kSgRows][65];
+  __shared__ float4 cpos[256];     // x, y, z, channel
+  const int2 blk = d.sg_rows[blockIdx.x];
+  const int sys = d.asys[blk.x];
+  if (!d.sys_sg[sys]) return;       // no model on this system (block-uniform)
+  const SysInfo S = d.sys[sys];
+  const int i = blk.x + threadIdx.x;
+  const bool row = i < blk.y;
+  for (int k = 0; k < 64; k++) F[threadIdx.x][k] = 0.f;
+  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (row) pi = d.pos[i];
+  for (int base = S.a0; base < S.a0 + S.nh; base += 256) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < 256; k += kSgRows) {
+      const int j = base + k;
+      if (j < S.a0 + S.nh) {
+        const float4 p = d.pos[j];
+        cpos[k] = make_float4(p.x, p.y, p.z, (float)sg_channel(p.w));
+      }
+    }
+    __syncthreads();
+    const int nc = min(256, S.a0 + S.nh - base);
+    if (row) {
+      for (int c = 0; c < nc; c++) {
+        const float4 pj = cpos[c];
+        const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        const float d2 = ex * ex + ey * ey + ez * ez;
+        if (!(d2 < kSgCut * kSgCut) || base + c == i) continue;
+        const float dist = sqrtf(d2);
+        float T, Tp;
+        sg_taper(dist, T, Tp);
+        const float u = dist * (1.f / kSgH) - 0.5f;
+        const int k0 = (int)floorf(u) - 1;
+        float *f = &F[threadIdx.x][(int)pj.w * 16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int k = k0 + q;
+          if (k >= 0 && k < kSgNShell) f[k] += bsp3(u - (float)k) * T;
+        }
+      }
+    }
+  }
+  if (!row) return;
+  // raw shell sums, normalised bf16 input row, in-range test
+  bool oor = false;
+  unsigned short *x = d.sg_X + (size_t)i * kSgKPad;
+  for (int k = 0; k < kSgKPad; k++) {
+    float v = 0.f;
+    if (k < kSgNFeat) {
+      const float raw = k < 64 ? F[threadIdx.x][k] : (k == 64 ? pi.w : d.q[i]);
+      if (k < 64) d.sg_F[(size_t)i * 64 + k] = raw;
+      v = (raw - d.sg.xmu[k]) * d.sg.xsd_inv[k];
+      oor |= !(v >= d.sg.xlo[k] && v <= d.sg.xhi[k]);
+    }
+    x[k] = __builtin_bit_cast(unsigned short, (__bf16)v);
+  }
+  if (oor) d.sg_state[4 * sys + 3] = 1;
+}
+
+// ---------------------------------------------------------------- the MLP
+// Z[32][128] = A[32][K] W + b with W staged transposed ([out][K]) in LDS
+template <int K, int LDA, int LDW>
+__device__ __forceinline__ void mlp_layer(const __bf16 *A, const __bf16 *Wt, const float *bias, float *Z) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int mt = t & 1, nt = 2 * w + (t >> 1);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      const bf16x8 a = *(const bf16x8 *)(A + (mt * 16 + (lane & 15)) * LDA + k0 + 8 * (lane >> 4));
+      const bf16x8 b = *(const bf16x8 *)(Wt + (nt * 16 + (lane & 15)) * LDW + k0 + 8 * (lane >> 4));
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int col = nt * 16 + (lane & 15);
+    const float bc = bias ? bias[col] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) Z[(mt * 16 + 4 * (lane >> 4) + r) * kSgHid + col] = acc[r] + bc;
+  }
+}
+
+// Out[32][NOUT] = D[32][128] W^T with W row-major [NOUT rows][128] in global memory
+template <int NOUT>
+__device__ __forceinline__ void mlp_back(const __bf16 *D, const unsigned short *W, float *Out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int NT = NOUT / 16;
+  for (int t = w; t < 2 * NT; t += 4) {
+    const int mt = t / NT, nt = t % NT;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < kSgHid; k0 += 32) {
+      const bf16x8 a = *(const bf16x8 *)(D + (mt * 16 + (lane & 15)) * kLdH + k0 + 8 * (lane >> 4));
+      const bf16x8 b = *(const bf16x8 *)(W + (size_t)(nt * 16 + (lane & 15)) * kSgHid + k0 + 8 * (lane >> 4));
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) Out[(mt * 16 + 4 * (lane >> 4) + r) * kSgHid + nt * 16 + (lane & 15)] = acc[r];
+  }
+}
+
+struct MlpLds {
+  __bf16 wt1[kSgHid * kLd1], wt2[kSgHid * kLdH], wt3[kSgHid * kLdH];
+  __bf16 act[kSgTile * kLdH];
+  float z1[kSgTile * kSgHid], z2[kSgTile * kSgHid], z3[kSgTile * kSgHid];
+  float part[kSgTile][8];
+  float seed[kSgTile];
+};
+
+__device__ void stage_weights(const SgModel &m, MlpLds &L) {
+  for (int k = threadIdx.x; k < kSgHid * kSgKPad; k += 256) {
+    const int o = k / kSgKPad, c = k - o * kSgKPad;
+    L.wt1[o * kLd1 + c] = __builtin_bit_cast(__bf16, m.wt1[k]);
+  }
+  for (int k = threadIdx.x; k < kSgHid * kSgHid; k += 256) {
+    const int o = k / kSgHid, c = k - o * kSgHid;
+    L.wt2[o * kLdH + c] = __builtin_bit_cast(__bf16, m.wt2[k]);
+    L.wt3[o * kLdH + c] = __builtin_bit_cast(__bf16, m.wt3[k]);
+  }
+}
+
+// Forward for rows [r0, r0 + 32) of X (bf16, kSgKPad columns), z1..z3 left
+// in LDS; returns the normalised output y for thread t < 32 (fixed-order sum)
+__device__ float mlp_forward_tile(const SgModel &m, MlpLds &L, const unsigned short *X, int r0, int nrows) {
+  for (int k = threadIdx.x; k < kSgTile * kSgKPad; k += 256) {
+    const int r = k / kSgKPad, c = k - r * kSgKPad;
+    L.act[r * kLd1 + c] = r < nrows ? __builtin_bit_cast(__bf16, X[(size_t)(r0 + r) * kSgKPad + c]) : (__bf16)0.f;
+  }
+  __syncthreads();
+  mlp_layer<kSgKPad, kLd1, kLd1>(L.act, L.wt1, m.b1, L.z1);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
+    L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)silu(L.z1[k]);
+  __syncthreads();
+  mlp_layer<kSgHid, kLdH, kLdH>(L.act, L.wt2, m.b2, L.z2);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
+    L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)silu(L.z2[k]);
+  __syncthreads();
+  mlp_layer<kSgHid, kLdH, kLdH>(L.act, L.wt3, m.b3, L.z3);
+  __syncthreads();
+  {
+    const int r = threadIdx.x >> 3, p = threadIdx.x & 7;
+    float ps = 0.f;
+    for (int c = 16 * p; c < 16 * p + 16; c++) ps += silu(L.z3[r * kSgHid + c]) * m.w4[c];
+    L.part[r][p] = ps;
+  }
+  __syncthreads();
+  float y = 0.f;
+  if (threadIdx.x < kSgTile) {
+    y = *m.b4;
+    for (int p = 0; p < 8; p++) y += L.part[threadIdx.x][p];
+  }
+  return y;
+}
+
+// delta3 = seed_r w4 silu'(z3) into act (seed[] set by the caller)
+__device__ void mlp_seed_delta3(const SgModel &m, MlpLds &L) {
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
+    const int r = k / kSgHid, c = k % kSgHid;
+    L.act[r * kLdH + c] = (__bf16)(L.seed[r] * m.w4[c] * silup(L.z3[k]));
+  }
+  __syncthreads();
+}
+
+// Back through the hidden layers from delta3 in act.  With `keep`, each
+// delta is first offered to keep(layer, act) (training stores them); the
+// input gradient d y / d X (first 64 columns) is left in z3 when `want_dx`.
+template <typename Keep>
+__device__ void mlp_backward_tile(const SgModel &m, MlpLds &L, bool want_dx, Keep keep) {
+  keep(3);
+  mlp_back<kSgHid>(L.act, m.w3, L.z3);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
+    L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)(L.z3[k] * silup(L.z2[k]));
+  __syncthreads();
+  keep(2);
+  mlp_back<kSgHid>(L.act, m.w2, L.z3);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
+    L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)(L.z3[k] * silup(L.z1[k]));
+  __syncthreads();
+  keep(1);
+  if (want_dx) {
+    mlp_back<64>(L.act, m.w1, L.z3);
+    __syncthreads();
+  }
+}
+
+// inference over the atoms of systems with a model: s_i and ds_i/dF_i
+__global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
+  const SgModel &m = d.sg;
+  stage_weights(m, L);
+  const int ntiles = (d.n + kSgTile - 1) / kSgTile;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int r0 = tile * kSgTile, nrows = min(kSgTile, d.n - r0);
+    __syncthreads();
+    const float y = mlp_forward_tile(m, L, d.sg_X, r0, nrows);
+    if (threadIdx.x < kSgTile) {
+      if (threadIdx.x < nrows) d.sg_s[r0 + threadIdx.x] = y * m.ysd + m.ymu;
+      L.seed[threadIdx.x] = 1.0f;
+    }
+    __syncthreads();
+    mlp_seed_delta3(m, L);
+    mlp_backward_tile(m, L, true, [](int) {});
+    for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
+      const int r = k >> 6, c = k & 63;
+      if (r < nrows) d.sg_dsdF[(size_t)(r0 + r) * 64 + c] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- training
+// One step over a batch of B rows of the normalised training set: forward,
+// d(mean squared error)/dy, the deltas of the three hidden layers, stored
+// feature-major (transposed) for the weight-gradient products.
+__global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r0, int B) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
+  stage_weights(m, L);
+  const int ntiles = (B + kSgTile - 1) / kSgTile;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int b0 = tile * kSgTile, nrows = min(kSgTile, B - b0);
+    __syncthreads();
+    const float y = mlp_forward_tile(m, L, t.X, r0 + b0, nrows);
+    if (threadIdx.x < kSgTile) {
+      float dy = 0.f;
+      if (threadIdx.x < nrows) {
+        const float e = y - t.y[r0 + b0 + threadIdx.x];
+        dy = 2.0f * e / (float)B;
+        t.dy[b0 + threadIdx.x] = dy;
+        t.err2[b0 + threadIdx.x] = e * e;
+      }
+      L.seed[threadIdx.x] = dy;
+    }
+    __syncthreads();
+    // activations (feature-major): X, a1, a2 as bf16; a3 in FP32
+    for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
+      const int r = k / kSgHid, c = k % kSgHid;
+      if (r >= nrows) continue;
+      const size_t col = (size_t)c * B + b0 + r;
+      t.A1[col] = __builtin_bit_cast(unsigned short, (__bf16)silu(L.z1[k]));
+      t.A2[col] = __builtin_bit_cast(unsigned short, (__bf16)silu(L.z2[k]));
+      t.A3[col] = silu(L.z3[k]);
+    }
+    for (int k = threadIdx.x; k < kSgTile * kSgKPad; k += 256) {
+      const int r = k / kSgKPad, c = k % kSgKPad;
+      if (r < nrows) t.A0[(size_t)c * B + b0 + r] = t.X[(size_t)(r0 + b0 + r) * kSgKPad + c];
+    }
+    mlp_seed_delta3(m, L);
+    mlp_backward_tile(m, L, false, [&](int layer) {
+      unsigned short *D = layer == 3 ? t.D3 : (layer == 2 ? t.D2 : t.D1);
+      for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
+        const int r = k / kSgHid, c = k % kSgHid;
+        if (r < nrows) D[(size_t)c * B + b0 + r] = __builtin_bit_cast(unsigned short, L.act[r * kLdH + c]);
+      }
+    });
+  }
+}
+
+// weight gradients dW_l = A_{l-1}^T delta_l: one wave per 16 x 16 tile,
+// K = the batch (feature-major operands make both MFMA loads contiguous)
+__global__ __launch_bounds__(64) void k_sg_train_dw(SgTrain t, float *grad, int B) {
+  const int lane = threadIdx.x;
+  int tile = blockIdx.x, layer, in_tiles;
+  if (tile < (kSgKPad / 16) * 8) { layer = 1; in_tiles = kSgKPad / 16; }
+  else if ((tile -= (kSgKPad / 16) * 8) < 64) { layer = 2; in_tiles = 8; }
+  else { tile -= 64; layer = 3; in_tiles = 8; }
+  (void)in_tiles;
+  const int it = tile / 8, ot = tile % 8;
+  const unsigned short *A = layer == 1 ? t.A0 : (layer == 2 ? t.A1 : t.A2);
+  const unsigned short *D = layer == 1 ? t.D1 : (layer == 2 ? t.D2 : t.D3);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < B; k0 += 32) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int k = k0 + 8 * (lane >> 4) + r;
+      const bool ok = k < B;
+      a[r] = ok ? __builtin_bit_cast(__bf16, A[(size_t)(it * 16 + (lane & 15)) * B + k]) : (__bf16)0.f;
+      b[r] = ok ? __builtin_bit_cast(__bf16, D[(size_t)(ot * 16 + (lane & 15)) * B + k]) : (__bf16)0.f;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+  const int off = layer == 1 ? kSgOffW1 : (layer == 2 ? kSgOffW2 : kSgOffW3);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int in = it * 16 + 4 * (lane >> 4) + r, out = ot * 16 + (lane & 15);
+    if (layer == 1 && in >= kSgNFeat) continue;
+    grad[off + in * kSgHid + out] = acc[r];
+  }
+}
+
+// bias and output-layer gradients (rows of the feature-major deltas), the
+// batch loss; one block per row, fixed-order sums
+__global__ __launch_bounds__(256) void k_sg_train_rows(SgTrain t, float *grad, float *loss, int B) {
+  __shared__ float red[256];
+  const int row = blockIdx.x;          // 0..127 b1, 128..255 b2, 256..383 b3, 384..511 w4, 512 b4, 513 loss
+  const float *src32 = nullptr;
+  const unsigned short *src16 = nullptr;
+  int dst;
+  if (row < 384) {
+    const int l = row / 128, c = row % 128;
+    src16 = (l == 0 ? t.D1 : (l == 1 ? t.D2 : t.D3)) + (size_t)c * B;
+    dst = (l == 0 ? kSgOffB1 : (l == 1 ? kSgOffB2 : kSgOffB3)) + c;
+  } else if (row < 512) {
+    dst = kSgOffW4 + row - 384;
+  } else {
+    dst = row == 512 ? kSgOffB4 : -1;
+    src32 = row == 512 ? t.dy : t.err2;
+  }
+  float s = 0.f;
+  for (int k = threadIdx.x; k < B; k += 256) {
+    float v;
+    if (src16) v = (float)__builtin_bit_cast(__bf16, src16[k]);
+    else if (row < 512) v = t.A3[(size_t)(row - 384) * B + k] * t.dy[k];
+    else v = src32[k];
+    s += v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (dst >= 0) grad[dst] = red[0];
+    else *loss = red[0] / (float)B;
+  }
+}
+
+// Adam on the FP32 master parameters, then the bf16 copies the kernels read
+__global__ __launch_bounds__(256) void k_sg_adam(float *p, const float *g, float *mom, float *vel, int n, float lr,
+                                                 float b1t, float b2t) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const float gk = g[k];
+  const float m = 0.9f * mom[k] + 0.1f * gk, v = 0.999f * vel[k] + 0.001f * gk * gk;
+  mom[k] = m;
+  vel[k] = v;
+  p[k] -= lr * (m / (1.f - b1t)) / (sqrtf(v / (1.f - b2t)) + 1e-8f);
+}
+
+__global__ __launch_bounds__(256) void k_sg_pack(const float *p, unsigned short *w1, unsigned short *w2,
+                                                 unsigned short *w3, unsigned short *wt1, unsigned short *wt2,
+                                                 unsigned short *wt3) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  auto bf = [](float v) { return __builtin_bit_cast(unsigned short, (__bf16)v); };
+  if (k < kSgKPad * kSgHid) {          // W1 padded to kSgKPad input rows
+    const int in = k / kSgHid, out = k % kSgHid;
+    const unsigned short v = in < kSgNFeat ? bf(p[kSgOffW1 + in * kSgHid + out]) : (unsigned short)0;
+    w1[k] = v;
+    wt1[out * kSgKPad + in] = v;
+  }
+  if (k < kSgHid * kSgHid) {
+    const int in = k / kSgHid, out = k % kSgHid;
+    const unsigned short a = bf(p[kSgOffW2 + k]), b = bf(p[kSgOffW3 + k]);
+    w2[k] = a;
+    w3[k] = b;
+    wt2[out * kSgHid + in] = a;
+    wt3[out * kSgHid + in] = b;
+  }
+}
+
+// ------------------------------------------------------ mode per system
+// anchor (plain) every `every` calls, before the first anchor and whenever an
+// atom's descriptors leave the training window; otherwise the surrogate
+__global__ void k_sg_mode(Dev d) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.nsys) return;
+  int *st = d.sg_state + 4 * b;    // [0] calls, [1] anchored, [2] mode (1 = surrogate), [3] out of range
+  if (!d.sys_sg[b]) { st[2] = 0; return; }
+  const int calls = st[0] + 1;
+  st[0] = calls;
+  const bool anchor = !st[1] || st[3] || d.sg_every <= 1 || (calls - 1) % d.sg_every == 0;
+  st[2] = anchor ? 0 : 1;
+}
+
+// ------------------------------------------------------- gradient pass
+// For surrogate-mode systems: row a collects -w d s_a / d x_a (a as centre,
+// heavy neighbours i) and -w_i d s_i / d x_a (a heavy, as neighbour of every
+// atom i), for w = q2ab (GB part, acc_ch 0..2) and w = abrw (vdW, 3..5).
+// d s_i / d x_a = sum_k dsdF_i[ch(a)][k] phi_k'(d) (x_a - x_i) / d.
+constexpr int kSgCols = 64;
+
+__global__ __launch_bounds__(kSgRows) void k_sg_grad(Dev d) {
+  __shared__ float G[kSgRows][65];          // own ds/dF
+  __shared__ float Gc[kSgCols][65];         // columns' ds/dF
+  __shared__ float4 cpos[kSgCols];          // x, y, z, channel (-1: hydrogen)
+  __shared__ float2 cw[kSgCols];            // q2ab, abrw
+  const int2 blk = d.sg_rows[blockIdx.x];
+  const int sys = d.asys[blk.x];
+  if (d.sg_state[4 * sys + 2] != 1) return;  // plain call for this system (block-uniform)
+  const SysInfo S = d.sys[sys];
+  const int a = blk.x + threadIdx.x;
+  const bool row = a < blk.y;
+  const bool aheavy = row && a < S.a0 + S.nh;
+  float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (row) {
+    pa = d.pos[a];
+    for (int k = 0; k < 64; k++) G[threadIdx.x][k] = d.sg_dsdF[(size_t)a * 64 + k];
+  }
+  const int cha = aheavy ? sg_channel(pa.w) : 0;
+  float sx = 0.f, sy = 0.f, sz = 0.f;                 // d s_a / d x_a
+  float gx = 0.f, gy = 0.f, gz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;   // neighbour terms
+  for (int base = S.a0; base < S.a0 + S.n; base += kSgCols) {
+    __syncthreads();
+    const int nc = min(kSgCols, S.a0 + S.n - base);
+    for (int k = threadIdx.x; k < nc * 64; k += kSgRows) {
+      const int c = k >> 6, f = k & 63;
+      Gc[c][f] = d.sg_dsdF[(size_t)(base + c) * 64 + f];
+    }
+    if (threadIdx.x < nc) {
+      const int j = base + threadIdx.x;
+      const float4 p = d.pos[j];
+      cpos[threadIdx.x] = make_float4(p.x, p.y, p.z, j < S.a0 + S.nh ? (float)sg_channel(p.w) : -1.f);
+      cw[threadIdx.x] = make_float2(d.q2ab_f[j], d.abrw_f[j]);
+    }
+    __syncthreads();
+    if (!row) continue;
+    for (int c = 0; c < nc; c++) {
+      const float4 pj = cpos[c];
+      const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;   // x_a - x_i
+      const float d2 = ex * ex + ey * ey + ez * ez;
+      if (!(d2 < kSgCut * kSgCut) || base + c == a) continue;
+      const bool iheavy = pj.w >= 0.f;
+      if (!iheavy && !aheavy) continue;
+      const float dist = sqrtf(d2), inv = 1.0f / dist;
+      float T, Tp;
+      sg_taper(dist, T, Tp);
+      const float u = dist * (1.f / kSgH) - 0.5f;
+      const int k0 = (int)floorf(u) - 1;
+      float cA = 0.f, cB = 0.f;
+      const int chi = iheavy ? (int)pj.w : 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int k = k0 + q;
+        if (k < 0 || k >= kSgNShell) continue;
+        const float t = u - (float)k;
+        const float dphi = bsp3p(t) * (1.f / kSgH) * T + bsp3(t) * Tp;
+        if (iheavy) cA += G[threadIdx.x][chi * 16 + k] * dphi;
+        if (aheavy) cB += Gc[c][cha * 16 + k] * dphi;
+      }
+      cA *= inv;
+      cB *= inv;
+      sx += cA * ex; sy += cA * ey; sz += cA * ez;
+      const float2 w = cw[c];
+      gx += w.x * cB * ex; gy += w.x * cB * ey; gz += w.x * cB * ez;
+      vx += w.y * cB * ex; vy += w.y * cB * ey; vz += w.y * cB * ez;
+    }
+  }
+  if (!row) return;
+  const float qa = d.q2ab_f[a], ba = d.abrw_f[a];
+  long long *p = d.acc_ch + 8 * (size_t)a;
+  fix_add(p + 0, -(double)(qa * sx + gx));
+  fix_add(p + 1, -(double)(qa * sy + gy));
+  fix_add(p + 2, -(double)(qa * sz + gz));
+  fix_add(p + 3, -(double)(ba * sx + vx));
+  fix_add(p + 4, -(double)(ba * sy + vy));
+  fix_add(p + 5, -(double)(ba * sz + vz));
+}
+
+// ------------------------------------------------------------- the judge
+// On anchor calls of a system that was anchored before: the first-order
+// change of Delta G_solv had the surrogate been used instead of the plain
+// kernel, sum_i (dE/ds_i) (s_i^surrogate - s_i^plain) with
+// dE/ds_i = -(332 df q2ab_i + abrw_i) / 4pi; then mark the system anchored.
+__global__ __launch_bounds__(256) void k_sg_judge(Dev d) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  int *st = d.sg_state + 4 * b;
+  if (!d.sys_sg[b] || st[2] != 0) return;     // block-uniform
+  const SysInfo S = d.sys[b];
+  const bool had = st[1] != 0;
+  double acc = 0.0;
+  if (had)
+    for (int i = S.a0 + threadIdx.x; i < S.a0 + S.n; i += 256)
+      acc += -(kTokcal * S.df64 * d.q2ab[i] + d.abrw[i]) / (4.0 * kPi) * d.sg_delta[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (had) d.sg_err[b] = red[0];
+    st[1] = 1;
+  }
+}
+
+// append the descriptors and plain targets of every atom to a training set
+__global__ __launch_bounds__(256) void k_sg_append(Dev d, float *F, float *target, int at) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= d.n) return;
+  float *f = F + (size_t)(at + i) * kSgNFeat;
+  for (int k = 0; k < 64; k++) f[k] = d.sg_F[(size_t)i * 64 + k];
+  f[64] = d.R[i];
+  f[65] = d.q[i];
+  target[at + i] = d.sg_target[i];
+}
+
+// ----------------------------------------------------------- launchers
+size_t sg_mlp_lds() { return sizeof(MlpLds); }
+
+
+void launch_sg_front(const Dev &d, hipStream_t s) {
+  if (!d.sg_on) return;
+  hipLaunchKernelGGL(k_sg_features, dim3(d.nsg_rows), dim3(kSgRows), 0, s, d);
+  hipLaunchKernelGGL(k_sg_mode, dim3((d.nsys + 63) / 64), dim3(64), 0, s, d);
+  const int ntiles = (d.n + kSgTile - 1) / kSgTile;
+  hipLaunchKernelGGL(k_sg_mlp, dim3(std::min(ntiles, d.sg_grid)), dim3(256), sizeof(MlpLds), s, d);
+}
+
+void launch_sg_train_step(const SgModel &m, const SgTrain &t, int r0, int B, float *grad, float *loss,
+                          int grid, hipStream_t s) {
+  const int ntiles = (B + kSgTile - 1) / kSgTile;
+  hipLaunchKernelGGL(k_sg_train_fb, dim3(std::min(ntiles, grid)), dim3(256), sizeof(MlpLds), s, m, t, r0, B);
+  hipLaunchKernelGGL(k_sg_train_dw, dim3((kSgKPad / 16) * 8 + 128), dim3(64), 0, s, t, grad, B);
+  hipLaunchKernelGGL(k_sg_train_rows, dim3(514), dim3(256), 0, s, t, grad, loss, B);
+}
+
+void launch_sg_adam(float *p, const float *g, float *mom, float *vel, float lr, int step, hipStream_t s) {
+  const float b1t = powf(0.9f, (float)step), b2t = powf(0.999f, (float)step);
+  hipLaunchKernelGGL(k_sg_adam, dim3((kSgNParam + 255) / 256), dim3(256), 0, s, p, g, mom, vel, kSgNParam, lr, b1t,
+                     b2t);
+}
+
+void launch_sg_pack(const float *p, unsigned short *w1, unsigned short *w2, unsigned short *w3, unsigned short *wt1,
+                    unsigned short *wt2, unsigned short *wt3, hipStream_t s) {
+  hipLaunchKernelGGL(k_sg_pack, dim3((kSgKPad * kSgHid + 255) / 256), dim3(256), 0, s, p, w1, w2, w3, wt1, wt2, wt3);
+}
+
+void launch_sg_judge(const Dev &d, hipStream_t s) {
+  if (d.sg_on) hipLaunchKernelGGL(k_sg_judge, dim3(d.nsys), dim3(256), 0, s, d);
+}
+
+void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t s) {
+  hipLaunchKernelGGL(k_sg_append, dim3((d.n + 255) / 256), dim3(256), 0, s, d, F, target, at);
+}
+
+// inference on given inputs (tests): X [n][kSgKPad] normalised bf16
+__global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const unsigned short *X, int n, float *s_out,
+                                                  float *dsdF) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
+  stage_weights(m, L);
+  const int ntiles = (n + kSgTile - 1) / kSgTile;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int r0 = tile * kSgTile, nrows = min(kSgTile, n - r0);
+    __syncthreads();
+    const float y = mlp_forward_tile(m, L, X, r0, nrows);
+    if (threadIdx.x < kSgTile) {
+      if (threadIdx.x < nrows) s_out[r0 + threadIdx.x] = y * m.ysd + m.ymu;
+      L.seed[threadIdx.x] = 1.0f;
+    }
+    __syncthreads();
+    mlp_seed_delta3(m, L);
+    mlp_backward_tile(m, L, true, [](int) {});
+    for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
+      const int r = k >> 6, c = k & 63;
+      if (r < nrows) dsdF[(size_t)(r0 + r) * 64 + c] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
+    }
+  }
+}
+
+void launch_sg_infer(const SgModel &m, const unsigned short *X, int n, float *s_out, float *dsdF, int grid,
+                     hipStream_t s) {
+  const int ntiles = (n + kSgTile - 1) / kSgTile;
+  hipLaunchKernelGGL(k_sg_infer, dim3(std::min(ntiles, grid)), dim3(256), sizeof(MlpLds), s, m, X, n, s_out, dsdF);
+}
+
+void launch_sg_grad(const Dev &d, hipStream_t s) {
+  if (!d.sg_on) return;
+  hipLaunchKernelGGL(k_sg_grad, dim3(d.nsg_rows), dim3(kSgRows), 0, s, d);
+}
+
+hipError_t prepare_surrogate() {
+  hipError_t e = hipFuncSetAttribute((const void *)k_sg_mlp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(MlpLds));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_sg_train_fb, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(MlpLds));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_sg_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(MlpLds));
+  return e;
+}
+
+}  // namespace agb

@@ -102,6 +102,47 @@ int agbnp3_batch_set_tuning(int batch, const int *params, int nparams);
 int agbnp3_batch_tune(int batch, const double *x, const double *y, const double *z, int reps, char *report,
                       int report_len);
 
+/* ---- learned Born-radius surrogate --------------------------------------- */
+/* A per-atom MLP (66 -> 128 -> 128 -> 128 -> 1, SiLU, bf16 on the matrix
+   cores with FP32 accumulation) on local descriptors (4 heavy-atom channels
+   x 16 radial shells to 12 A, R, q) predicts the descreening sum of the
+   inverse Born radius.  Attached to an instance or batch with anchor period
+   `every`, a system runs the plain kernels on its first call, every `every`
+   calls and whenever an atom's descriptors leave the training window
+   ("anchor" calls); in between its descreening sum is the model's plus the
+   correction the last anchor measured, with the model's gradient in place of
+   the descreening chain rule.  On each anchor the first-order error of the
+   solvation energy the surrogate would have made is recorded
+   (agbnp3_*sg_stats).  No counterpart in the reference. */
+int agbnp3_sg_new(int *model, unsigned long long seed);
+int agbnp3_sg_delete(int model);
+/* flat export: parameters, feature normalisation and window, target scale */
+int agbnp3_sg_size(void);
+int agbnp3_sg_get_params(int model, float *out, int n);
+int agbnp3_sg_set_params(int model, const float *in, int n);
+/* training set: descriptors and plain descreening sums of every atom of a
+   batch evaluated (plain) at device coordinates x, y, z */
+int agbnp3_sg_reserve(int model, int natoms);
+int agbnp3_batch_sg_collect(int batch, int model, const double *x, const double *y, const double *z);
+/* count, raw descriptors [count][66] and targets [count] (may be NULL) */
+int agbnp3_sg_dataset(int model, int *count, float *F, float *target);
+/* normalisation, training window (range widened by `margin` of its width),
+   shuffled bf16 training matrix */
+int agbnp3_sg_fit(int model, float margin, unsigned long long seed);
+/* Adam, mini-batches of B rows, lr decay^epoch; mean loss per epoch */
+int agbnp3_sg_train(int model, int epochs, int B, float lr, float decay, unsigned long long seed,
+                    float *epoch_loss);
+/* gradients of the mean squared error over rows [r0, r0 + B) (no update) */
+int agbnp3_sg_grads(int model, int r0, int B, float *grad, float *loss);
+/* the model on raw descriptors F [n][66]: s [n], ds/dF [n][64] */
+int agbnp3_sg_infer(int model, const float *F, int n, float *s, float *dsdF);
+int agbnp3_sg_attach(int tag, int model, int every);
+int agbnp3_batch_sg_attach(int batch, int model, int every);
+/* per system: error estimate of the last judged anchor, mode of the last call
+   (1 = surrogate) */
+int agbnp3_sg_stats(int tag, double *err, int *mode);
+int agbnp3_batch_sg_stats(int batch, double *err, int *mode);
+
 /* ---- host-only helpers (no GPU needed) ------------------------------------ */
 /* Q4 spline tables for radius types `radii` (R = r + 0.5): y, y2 sized
    ntypes*ntypes*512, table (i, j) at offset (i*ntypes + j)*512; dx[0..1] =

@@ -1,0 +1,211 @@
+"""The learned Born-radius surrogate on the GPU (agbnp3_sg_*) against its CPU
+reference (oracle/surrogate.py) and against the plain kernels.
+
+- descriptors of the GPU feature kernel vs the FP64 brute-force reference;
+- training targets vs the FP64 oracle's descreening sums;
+- MLP inference (bf16 MFMA) and its input gradient vs the bf16-mirroring
+  CPU reference; training gradients of the whole set vs the reference;
+- anchor calls equal the plain kernels bitwise; surrogate calls' gradients of
+  E_GB and E_vdW equal central differences of their energies;
+- training lowers the loss, and between anchors the trained network tracks
+  the plain solvation energy better than holding the anchor's radii fixed.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import surrogate as ref
+from agbnp3_amd import api, hip, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def concat(ss, k):
+    return np.concatenate([getattr(s, k) for s in ss])
+
+
+def dev_coords(x, y, z):
+    return [hip.DeviceArray.from_numpy(np.ascontiguousarray(a, np.float64)) for a in (x, y, z)]
+
+
+def collect(model, ss, frames=1, sigma=0.0, seed=0):
+    insts = [api.Instance(s) for s in ss]
+    b = api.Batch(insts)
+    base = [concat(ss, k) for k in ("x", "y", "z")]
+    try:
+        for f in range(frames):
+            rng = np.random.default_rng(seed + f)
+            xyz = [a + (rng.normal(0.0, sigma, a.size) if sigma else 0.0) for a in base]
+            d = dev_coords(*xyz)
+            model.collect(b, d[0].addr, d[1].addr, d[2].addr)
+            for a in d:
+                a.free()
+    finally:
+        b.delete()
+        for i in insts:
+            i.delete()
+
+
+def cpu_params(model):
+    """The GPU model's parameters and normalisation as the CPU reference's Params."""
+    flat = model.get_params().astype(np.float64)
+    parts, o = {}, 0
+    for name, shape in api.Surrogate.LAYOUT:
+        size = int(np.prod(shape))
+        parts[name] = flat[o:o + size].reshape(shape)
+        o += size
+    norm = flat[o:o + 4 * 66].reshape(4, 66)
+    ymu, ysd = flat[o + 4 * 66], flat[o + 4 * 66 + 1]
+    return ref.Params(parts["W1"], parts["b1"], parts["W2"], parts["b2"], parts["W3"], parts["b3"], parts["w4"],
+                      parts["b4"][0], norm[0], 1.0 / norm[1], ymu, ysd)
+
+
+@pytest.fixture(scope="module")
+def small_set():
+    ss = [synth.generate(20264000 + k, n) for k, n in enumerate((350, 420, 510))]
+    m = api.Surrogate(seed=1)
+    m.reserve(4096)
+    collect(m, ss)
+    m.fit(margin=0.25, seed=3)
+    yield ss, m
+    m.delete()
+
+
+def test_descriptors_and_targets_match_the_references(small_set):
+    ss, m = small_set
+    F, t = m.dataset()
+    o = 0
+    for s in ss:
+        orc = oracle.Oracle(s)
+        orc.ener()
+        i2e = orc.int2ext()
+        R = (s.r + 0.5).astype(np.float32).astype(np.float64)
+        pos = np.stack([s.x, s.y, s.z], 1).astype(np.float32).astype(np.float64)
+        heavy = ~np.isin(np.arange(s.natoms), s.ihydrogen)
+        Fr = ref.features(pos[i2e], R[i2e], s.charge.astype(np.float32)[i2e], heavy[i2e])
+        Fg = F[o:o + s.natoms].astype(np.float64)
+        assert np.allclose(Fg[:, :64], Fr[:, :64], rtol=1e-5, atol=1e-4)
+        assert np.array_equal(Fg[:, 64], R[i2e].astype(np.float32))
+        beta = orc.internal("beta")
+        s_ref = 4.0 * np.pi * (1.0 / (R[i2e] - 0.5) - beta)
+        assert np.allclose(t[o:o + s.natoms], s_ref, rtol=1e-5, atol=1e-5)
+        o += s.natoms
+
+
+def test_mlp_inference_matches_cpu_reference(small_set):
+    ss, m = small_set
+    F, _ = m.dataset()
+    P = cpu_params(m)
+    s_gpu, d_gpu = m.infer(F)
+    s_ref, cache = ref.forward(P, F.astype(np.float64))
+    d_ref = ref.input_gradient(P, cache)[:, :64]
+    # bf16 operands and activations; FP32 vs exact accumulation
+    assert np.sqrt(np.mean((s_gpu - s_ref) ** 2)) <= 1e-3 * max(1.0, np.std(s_ref))
+    assert np.max(np.abs(s_gpu - s_ref)) <= 1e-2 * max(1.0, np.std(s_ref))
+    assert np.sqrt(np.mean((d_gpu - d_ref) ** 2)) <= 1e-2 * np.sqrt(np.mean(d_ref ** 2))
+
+
+def test_training_gradients_match_cpu_reference(small_set):
+    ss, m = small_set
+    F, t = m.dataset()
+    P = cpu_params(m)
+    n = len(t)
+    loss_g, g = m.grads(0, n)                         # all rows: order-independent sums
+    loss_r, gr = ref.loss_and_weight_gradients(P, F.astype(np.float64), t.astype(np.float64))
+    assert abs(loss_g - loss_r) <= 1e-3 * loss_r
+    o = 0
+    for name, shape in api.Surrogate.LAYOUT:
+        size = int(np.prod(shape))
+        a, b = g[o:o + size], np.ravel(gr[name])
+        assert np.sqrt(np.mean((a - b) ** 2)) <= 2e-2 * np.sqrt(np.mean(b ** 2)) + 1e-9, name
+        o += size
+
+
+def test_anchor_calls_equal_the_plain_kernels(small_set):
+    ss, m = small_set
+    s = ss[1]
+    plain, sur = api.Instance(s), api.Instance(s)
+    try:
+        sur.sg_attach(m, 3)
+        for call in range(4):
+            rng = np.random.default_rng(20263100 + call)
+            x, y, z = (a + rng.normal(0.0, 0.05, a.size) for a in (s.x, s.y, s.z))
+            a, b = sur.ener(x, y, z), plain.ener(x, y, z)
+            err, mode = sur.sg_stats()
+            if call in (0, 3):
+                assert mode == 0
+                for k in ("dgbdr", "dvwdr", "decav", "dehb", "br") + api.ENERGY_NAMES:
+                    assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), (call, k)
+            else:
+                assert mode == 1
+    finally:
+        plain.delete()
+        sur.delete()
+
+
+def test_surrogate_gradients_are_derivatives_of_its_energy(small_set):
+    ss, m = small_set
+    s = ss[0]
+    inst = api.Instance(s)
+    try:
+        inst.sg_attach(m, 1000)                       # one anchor, then surrogate calls only
+        inst.ener()
+        out = inst.ener()
+        assert inst.sg_stats()[1] == 1
+        rng = np.random.default_rng(9)
+        h = 1e-2
+        for a in rng.choice(s.natoms, 4, replace=False):
+            for c in range(3):
+                xyz = [s.x.copy(), s.y.copy(), s.z.copy()]
+                xyz[c][a] += h
+                ep = inst.ener(*xyz)
+                xyz[c][a] -= 2 * h
+                em = inst.ener(*xyz)
+                for e, g in (("egb", "dgbdr"), ("evdw", "dvwdr")):
+                    fd = (ep[e] - em[e]) / (2 * h)
+                    an = out[g][a, c]
+                    assert abs(fd - an) <= 2e-2 * max(1.0, abs(an)), (a, c, e, fd, an)
+    finally:
+        inst.delete()
+
+
+def test_training_and_the_gain_over_frozen_radii():
+    """Train on config-2-style conformations (seeds 20264000 + m) with the
+    GPU kernels; on a held-out solute, between anchors the network tracks
+    the plain Delta G_solv better than keeping the anchor's radii."""
+    sizes = synth.sizes(20264999, 24, 300, 3000)
+    ss = [synth.generate(20264000 + k, int(n)) for k, n in enumerate(sizes)]
+    m = api.Surrogate(seed=11)
+    frozen = api.Surrogate(seed=12)
+    try:
+        m.reserve(int(sizes.sum()) * 3)
+        collect(m, ss, frames=3, sigma=0.05, seed=20264500)
+        m.fit(margin=0.1, seed=5)
+        losses = m.train(epochs=25, batch=2048, lr=2e-3, decay=0.9, seed=7)
+        assert losses[-1] < 0.5 * losses[0]
+        # the method without the network: zero weights, the anchor's radii held
+        p = m.get_params()
+        p[:api.Surrogate.NPARAM] = 0.0
+        frozen.set_params(p)
+        held = synth.generate(20265000, 3000)
+        runs = {}
+        for name, model in (("net", m), ("frozen", frozen)):
+            plain, sur = api.Instance(held), api.Instance(held)
+            sur.sg_attach(model, 10)
+            errs = []
+            for call in range(10):
+                rng = np.random.default_rng(20265100 + call)
+                x, y, z = (a + rng.normal(0.0, 0.05, a.size) for a in (held.x, held.y, held.z))
+                a, b = sur.ener(x, y, z), plain.ener(x, y, z)
+                dga = sum(a[k] for k in ("egb", "evdw", "ecorr_vdw", "ecav", "ecorr_cav", "ehb"))
+                dgb = sum(b[k] for k in ("egb", "evdw", "ecorr_vdw", "ecav", "ecorr_cav", "ehb"))
+                errs.append(abs(dga - dgb) / abs(dgb))
+            plain.delete()
+            sur.delete()
+            runs[name] = np.array(errs)
+        assert runs["net"][0] == 0.0 and runs["frozen"][0] == 0.0       # anchor call
+        assert np.mean(runs["net"][1:]) < np.mean(runs["frozen"][1:]), runs
+        assert np.max(runs["net"][1:]) <= 1e-3, runs                     # Delta G_solv within 0.1 %
+    finally:
+        m.delete()
+        frozen.delete()
