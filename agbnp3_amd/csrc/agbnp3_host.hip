@@ -1864,10 +1864,15 @@ int agbnp3_sg_fit(int model, float margin, unsigned long long seed) {
       lo[k] = std::min(lo[k], v);
       hi[k] = std::max(hi[k], v);
     }
+  // the window also keeps an absolute floor in raw units (half an atom in a
+  // shell, 0.02 A of radius, 0.1 e of charge), so features that are rare in
+  // the training set do not reject every small displacement
   for (int k = 0; k < kSgNFeat; k++) {
+    const float floor_raw = k < 64 ? 0.5f : (k == 64 ? 0.02f : 0.1f);
     const float w = std::max(hi[k] - lo[k], 1e-3f);
-    norm[2 * kSgNFeat + k] = lo[k] - margin * w;
-    norm[3 * kSgNFeat + k] = hi[k] + margin * w;
+    const float pad = margin * w + floor_raw * norm[kSgNFeat + k];
+    norm[2 * kSgNFeat + k] = lo[k] - pad;
+    norm[3 * kSgNFeat + k] = hi[k] + pad;
   }
   M->ymu = (float)ym;
   M->ysd = (float)std::max(sqrt(ys / N), 1e-6);

@@ -599,7 +599,8 @@ void launch_sg_adam(float *p, const float *g, float *mom, float *vel, float lr, 
 
 void launch_sg_pack(const float *p, unsigned short *w1, unsigned short *w2, unsigned short *w3, unsigned short *wt1,
                     unsigned short *wt2, unsigned short *wt3, hipStream_t s) {
-  hipLaunchKernelGGL(k_sg_pack, dim3((kSgKPad * kSgHid + 255) / 256), dim3(256), 0, s, p, w1, w2, w3, wt1, wt2, wt3);
+  constexpr int n = std::max(kSgKPad * kSgHid, kSgHid * kSgHid);      // both weight shapes
+  hipLaunchKernelGGL(k_sg_pack, dim3((n + 255) / 256), dim3(256), 0, s, p, w1, w2, w3, wt1, wt2, wt3);
 }
 
 void launch_sg_judge(const Dev &d, hipStream_t s) {

@@ -126,8 +126,9 @@ int agbnp3_sg_reserve(int model, int natoms);
 int agbnp3_batch_sg_collect(int batch, int model, const double *x, const double *y, const double *z);
 /* count, raw descriptors [count][66] and targets [count] (may be NULL) */
 int agbnp3_sg_dataset(int model, int *count, float *F, float *target);
-/* normalisation, training window (range widened by `margin` of its width),
-   shuffled bf16 training matrix */
+/* normalisation, training window (range widened by `margin` of its width
+   plus half an atom per shell, 0.02 A of radius, 0.1 e of charge), shuffled
+   bf16 training matrix */
 int agbnp3_sg_fit(int model, float margin, unsigned long long seed);
 /* Adam, mini-batches of B rows, lr decay^epoch; mean loss per epoch */
 int agbnp3_sg_train(int model, int epochs, int B, float lr, float decay, unsigned long long seed,
