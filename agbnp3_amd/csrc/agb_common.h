@@ -189,6 +189,8 @@ struct Dev {
   long long *acc_gbA, *acc_gbg;        // GB pairs: A_i [n], gradient [n][3]
   long long *acc_ch;                   // chain rule [n][8]: gb xyz, vdw xyz, U, V
   unsigned long long *counters;        // [0] kept overlaps [1] evaluated [2] nbl entries [3] max row
+                                       // [4], [5] roots queued for the depth-first tree (pass A, B)
+  int *tree_ovf;                       // [2][nh] those roots' heavy rows
   double *volumep_raw, *sa_raw, *sa_f, *sp, *spe, *psvol, *gammap;   // [n]
   float *sp_f, *spe_f;                 // FP32 copies for pair kernels
   double *beta, *br, *brw, *swf;       // [n]

@@ -9,14 +9,18 @@
 // coefficient is (-1)^(n+1)/n for the volume and (-1)^(n+1) for the surface.
 //
 // GPU mapping: one wavefront per root.  Its neighbour row ("slots": 0 = the
-// root, 1..m = list entries) is staged in LDS; the tree is walked depth-first
-// over sibling groups, each group evaluated lane-parallel and compacted in
-// order with ballots.  Every overlap of the root involves only slot atoms,
-// so contributions are reduced per slot in LDS and scattered to atoms once
-// per root with order-independent fixed-point atomics.  Within the wave,
-// member contributions are added to FP64 LDS slots with ds_add_f64; only the
-// root, a member of every overlap, is summed in lane registers and reduced
-// once per root.
+// root, 1..m = list entries) is staged in LDS and the tree is built breadth
+// first: every candidate of one order -- for each node, the last atoms of
+// its later siblings -- is one lane-parallel pass (a few passes of 64 per
+// order instead of one per sibling group), kept overlaps are compacted in
+// candidate order with ballots, so each node's children stay contiguous and
+// in the order the reference forms them.  Every overlap of the root involves
+// only slot atoms, so contributions are reduced per slot in LDS and
+// scattered to atoms once per root with order-independent fixed-point
+// atomics; the root, a member of every overlap, is summed in lane registers.
+// A root whose tree outgrows the LDS level buffers is queued for the
+// depth-first kernel (one sibling group per pass), which holds one group per
+// order at a time.
 //
 // Pass B re-walks the identical tree (same FP32 arithmetic, same pruning)
 // instead of storing it, and applies in one place the derivatives of
@@ -83,6 +87,84 @@ __device__ __forceinline__ float sa_term(const OvlOut &o, float sign, float a, f
   const float ex = x - o.cx, ey = y - o.cy, ez = z - o.cz;
   const float t = ex * ex + ey * ey + ez * ez;
   return sign * o.fp * a * o.V * (3.f * o.dinv + 2.f * t) / R;
+}
+
+// Contributions of one kept overlap o of order N with members mem[0..N-1]
+// (mem[0] = 0, the root) to the root's registers and the slot accumulators.
+template <int N, int CAP, bool PB, typename Lds>
+__device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (&mem)[N], double (&root)[TG_N],
+                                            float df) {
+  const float sign = ovl_sign(N);
+  const float cn = sign / (float)N;
+  if (!PB) {
+    const double wv = (double)cn * (double)o.Vp;
+    root[0] += wv;
+    root[1] += (double)sa_term(o, sign, S.a[0], S.x[0], S.y[0], S.z[0], S.R[0]);
+#pragma unroll
+    for (int k = 1; k < N; k++) {
+      const int sk = mem[k];
+      atomicAdd(&S.acc[0][sk], wv);
+      atomicAdd(&S.acc[1][sk], (double)sa_term(o, sign, S.a[sk], S.x[sk], S.y[sk], S.z[sk], S.R[sk]));
+    }
+  } else {
+    float A_[N], X_[N], Y_[N], Z_[N], R_[N], WC[N], WV[N], WG[N];
+    float Us = 0.f, Vs = 0.f, Hs = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const int sk = mem[k];
+      A_[k] = S.a[sk]; X_[k] = S.x[sk]; Y_[k] = S.y[sk]; Z_[k] = S.z[sk]; R_[k] = S.R[sk];
+      WC[k] = sign * S.wc[sk]; WV[k] = sign * S.wv[sk]; WG[k] = sign * S.wg[sk];
+      Us += S.u[sk]; Vs += S.v[sk]; Hs += S.h[sk];
+    }
+    const float fourpi1 = 1.0f / (4.0f * kPiF);
+    const float Uc = -df * fourpi1 * cn * Us * o.fp;
+    const float Vc = -fourpi1 * cn * Vs * o.fp;
+    const float Hc = -cn * Hs * o.fp;
+    const float h3 = o.fpp + o.fp / o.V, h4 = 2.f * o.fp * o.dinv;
+    float gdr[N][3], gdR[N];
+    float S1c = 0.f, S1v = 0.f, S1g = 0.f;
+    float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const float ex = X_[k] - o.cx, ey = Y_[k] - o.cy, ez = Z_[k] - o.cz;
+      const float av = A_[k] * o.V;
+      gdr[k][0] = -2.f * av * ex;
+      gdr[k][1] = -2.f * av * ey;
+      gdr[k][2] = -2.f * av * ez;
+      gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) / R_[k];
+      S1c += WC[k] * gdR[k]; S1v += WV[k] * gdR[k]; S1g += WG[k] * gdR[k];
+      const float iR = 1.0f / R_[k];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        S2c[q] += WC[k] * gdr[k][q] * iR;
+        S2v[q] += WV[k] * gdr[k][q] * iR;
+        S2g[q] += WG[k] * gdr[k][q] * iR;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const float iR = 1.0f / R_[k];
+      const float t = -2.f * o.fp * (1.f - A_[k] * o.dinv) * iR;
+      const float bc = h3 * S1c + WC[k] * t, bv = h3 * S1v + WV[k] * t, bg = h3 * S1g + WG[k] * t;
+      const float ha = h4 * A_[k];
+      float out[TG_N];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const float g = gdr[k][q];
+        out[TG_GB + q] = g * bg + ha * (S2g[q] - WG[k] * g * iR) + Uc * g;
+        out[TG_VW + q] = g * bv + ha * (S2v[q] - WV[k] * g * iR) + Vc * g;
+        out[TG_CAV + q] = g * bc + ha * (S2c[q] - WC[k] * g * iR);
+        out[TG_HB + q] = Hc * g;
+      }
+      if (k == 0) {
+#pragma unroll
+        for (int v = 0; v < TG_N; v++) root[v] += (double)out[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < TG_N; v++) atomicAdd(&S.acc[v][mem[k]], (double)out[v]);
+      }
+    }
+  }
 }
 
 // Evaluate the children of one tree node: lanes take candidate slots, keep
@@ -215,11 +297,14 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
   return cnt;
 }
 
+// depth-first walk of the roots the breadth-first kernel queued (overflow);
+// the grid covers every heavy row, blocks past the queue length exit
 template <int CAP, bool PB>
 __global__ __launch_bounds__(64) void k_tree(Dev d) {
   __shared__ TreeLds<CAP, PB> S;
   const int lane = threadIdx.x;
-  const int h = blockIdx.x;
+  if (blockIdx.x >= (unsigned)d.counters[4 + (PB ? 1 : 0)]) return;
+  const int h = d.tree_ovf[(PB ? d.nh : 0) + blockIdx.x];
   const int i = d.hrow[h];
   const int m = d.nne[h];
   // stage the slot atoms
@@ -287,18 +372,227 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
   }
 }
 
+// ------------------------------------------------------ breadth-first tree
+constexpr int kLevelCap = 256;          // nodes of one order held in LDS
+
+template <int CAP, bool PB>
+struct BfsLds {
+  int idx[CAP + 1];
+  float a[CAP + 1], x[CAP + 1], y[CAP + 1], z[CAP + 1], R[CAP + 1];
+  float u[PB ? CAP + 1 : 1], v[PB ? CAP + 1 : 1], h[PB ? CAP + 1 : 1];
+  float wc[PB ? CAP + 1 : 1], wv[PB ? CAP + 1 : 1], wg[PB ? CAP + 1 : 1];
+  double acc[PB ? TG_N : 2][CAP + 1];
+  // two node buffers: Gaussian, member slots (16 bits each, root excluded),
+  // end of the node's sibling group, parent index in the previous order
+  float ga[2][kLevelCap], gp[2][kLevelCap], gx[2][kLevelCap], gy[2][kLevelCap], gz[2][kLevelCap];
+  unsigned long long path[2][kLevelCap];
+  int gend[2][kLevelCap], gpar[2][kLevelCap];
+  int pref[kLevelCap + 1], cnt[kLevelCap];
+  int ovf;
+};
+
+// exclusive scan of vals[0..n) into out[0..n], out[n] = total (one wave)
+__device__ __forceinline__ int wave_exclusive_scan(const int *vals, int *out, int n) {
+  const int lane = threadIdx.x & 63;
+  int run = 0;
+  for (int base = 0; base < n; base += 64) {
+    const int k = base + lane;
+    const int v = k < n ? vals[k] : 0;
+    const int incl = wave_incl_scan(v);
+    if (k < n) out[k] = run + incl - v;
+    run += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) out[n] = run;
+  return run;
+}
+
+// children of order N: pairs (u, later sibling v) of the nodes of order N-1
+// in buffer `cb` (n nodes), kept ones into buffer 1-cb; returns their count
+// (kLevelCap + 1 on overflow)
+template <int N, int CAP, bool PB, bool STORE>
+__device__ __forceinline__ int bfs_order(BfsLds<CAP, PB> &S, int cb, int n, double (&root)[TG_N], float df,
+                                         unsigned &nkept, unsigned &neval) {
+  const int lane = threadIdx.x & 63;
+  const int nb = 1 - cb;
+  for (int k = lane; k < n; k += 64) {
+    S.cnt[k] = S.gend[cb][k] - k - 1;     // later siblings of node k
+  }
+  __syncthreads();
+  const int P = wave_exclusive_scan(S.cnt, S.pref, n);
+  for (int k = lane; k < n; k += 64) S.cnt[k] = 0;
+  __syncthreads();
+  int nn = 0;
+  for (int base = 0; base < P; base += 64) {
+    const int q = base + lane;
+    bool keep = false;
+    int uu = 0, sb = 0;
+    OvlOut o;
+    if (q < P) {
+      int lo = 0, hi = n - 1;              // last u with pref[u] <= q
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S.pref[mid] <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      uu = lo;
+      const int vv = uu + 1 + (q - S.pref[uu]);
+      sb = (int)((S.path[cb][vv] >> (16 * (N - 3))) & 0xFFFFull);    // last atom of sibling v
+      const Gauss par = {S.ga[cb][uu], S.gp[cb][uu], S.gx[cb][uu], S.gy[cb][uu], S.gz[cb][uu]};
+      o = overlap(par, S.a[sb], S.x[sb], S.y[sb], S.z[sb]);
+      keep = o.Vp > FLT_MIN;
+    }
+    neval += (unsigned)__popcll(__ballot(q < P));
+    const unsigned long long bal = __ballot(keep);
+    if (keep) {
+      const unsigned long long pu = S.path[cb][uu];
+      int mem[N];
+      mem[0] = 0;
+#pragma unroll
+      for (int k = 0; k < N - 2; k++) mem[1 + k] = (int)((pu >> (16 * k)) & 0xFFFFull);
+      mem[N - 1] = sb;
+      add_contrib<N, CAP, PB>(S, o, mem, root, df);
+      if (STORE) {
+        const int pos = nn + __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < kLevelCap) {
+          S.ga[nb][pos] = o.a; S.gp[nb][pos] = o.p;
+          S.gx[nb][pos] = o.cx; S.gy[nb][pos] = o.cy; S.gz[nb][pos] = o.cz;
+          S.path[nb][pos] = pu | ((unsigned long long)sb << (16 * (N - 2)));
+          S.gpar[nb][pos] = uu;
+        }
+        atomicAdd(&S.cnt[uu], 1);
+      }
+    }
+    nn += __popcll(bal);
+  }
+  nkept += (unsigned)nn;
+  __syncthreads();
+  if (!STORE) return 0;
+  if (nn > kLevelCap) return kLevelCap + 1;
+  // a node's siblings are the kept children of its parent, contiguous
+  wave_exclusive_scan(S.cnt, S.pref, n);
+  __syncthreads();
+  for (int k = lane; k < nn; k += 64) {
+    const int par = S.gpar[nb][k];
+    S.gend[nb][k] = S.pref[par] + S.cnt[par];
+  }
+  __syncthreads();
+  return nn;
+}
+
+template <int CAP, bool PB>
+__global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
+  __shared__ BfsLds<CAP, PB> S;
+  const int lane = threadIdx.x;
+  const int h = blockIdx.x;
+  const int i = d.hrow[h];
+  const int m = d.nne[h];
+  for (int s = lane; s <= m; s += 64) {
+    const int at = s == 0 ? i : d.nbr[(size_t)h * d.cap + s - 1];
+    const float4 p = d.pos[at];
+    S.idx[s] = at;
+    S.a[s] = d.galpha[at];
+    S.x[s] = p.x; S.y[s] = p.y; S.z[s] = p.z; S.R[s] = p.w;
+    if (PB) {
+      S.u[s] = d.deru_f[at]; S.v[s] = d.derv_f[at]; S.h[s] = d.derh_f[at];
+      S.wc[s] = d.gammap_f[at]; S.wv[s] = d.dervs_f[at]; S.wg[s] = d.derus_f[at];
+    }
+#pragma unroll
+    for (int v = 0; v < (PB ? TG_N : 2); v++) S.acc[v][s] = 0.0;
+  }
+  __syncthreads();
+  double root[TG_N];
+#pragma unroll
+  for (int v = 0; v < TG_N; v++) root[v] = 0.0;
+  unsigned nkept = 0, neval = 0;
+  const float df = d.sys[d.asys[i]].df;
+  bool ovf = false;
+  // order 2: the root with every list entry, one sibling group
+  int n = 0;
+  {
+    const Gauss rg = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
+    for (int base = 0; base < m; base += 64) {
+      const int c = base + lane;
+      bool keep = false;
+      int sb = 0;
+      OvlOut o;
+      if (c < m) {
+        sb = c + 1;
+        o = overlap(rg, S.a[sb], S.x[sb], S.y[sb], S.z[sb]);
+        keep = o.Vp > FLT_MIN;
+      }
+      neval += (unsigned)__popcll(__ballot(c < m));
+      const unsigned long long bal = __ballot(keep);
+      if (keep) {
+        const int mem[2] = {0, sb};
+        add_contrib<2, CAP, PB>(S, o, mem, root, df);
+        const int pos = n + __popcll(bal & ((1ull << lane) - 1ull));
+        if (pos < kLevelCap) {
+          S.ga[0][pos] = o.a; S.gp[0][pos] = o.p;
+          S.gx[0][pos] = o.cx; S.gy[0][pos] = o.cy; S.gz[0][pos] = o.cz;
+          S.path[0][pos] = (unsigned long long)sb;
+        }
+      }
+      n += __popcll(bal);
+    }
+    nkept += (unsigned)n;
+    if (n > kLevelCap) ovf = true;
+    for (int k = lane; k < n && k < kLevelCap; k += 64) S.gend[0][k] = n;
+    __syncthreads();
+  }
+  if (!ovf && n > 1) {
+    n = bfs_order<3, CAP, PB, true>(S, 0, n, root, df, nkept, neval);
+    if (n > kLevelCap) ovf = true;
+    else if (n > 1) {
+      n = bfs_order<4, CAP, PB, true>(S, 1, n, root, df, nkept, neval);
+      if (n > kLevelCap) ovf = true;
+      else if (n > 1) bfs_order<5, CAP, PB, false>(S, 0, n, root, df, nkept, neval);
+    }
+  }
+  if (ovf) {                              // wave-uniform: the depth-first kernel takes this root
+    if (lane == 0) {
+      const unsigned k = atomicAdd(&d.counters[4 + (PB ? 1 : 0)], 1ull);
+      d.tree_ovf[(PB ? d.nh : 0) + k] = h;
+    }
+    return;
+  }
+#pragma unroll
+  for (int v = 0; v < (PB ? TG_N : 2); v++) {
+    const double r = wave_sum(root[v]);
+    if (lane == 0) S.acc[v][0] += r;
+  }
+  __syncthreads();
+  for (int s = lane; s <= m; s += 64) {
+    const int at = S.idx[s];
+    if (!PB) {
+      fix_add(&d.acc_vol[at], S.acc[0][s]);
+      fix_add(&d.acc_sa[at], S.acc[1][s]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], S.acc[v][s]);
+    }
+  }
+  if (!PB && lane == 0) {
+    atomicAdd(&d.counters[0], (unsigned long long)nkept);
+    atomicAdd(&d.counters[1], (unsigned long long)neval);
+  }
+}
+
+template <int CAP>
+static void launch_tree_cap(const Dev &d, int pass, hipStream_t s) {
+  if (pass == 0) {
+    hipLaunchKernelGGL((k_tree_bfs<CAP, false>), dim3(d.nh), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree<CAP, false>), dim3(d.nh), dim3(64), 0, s, d);
+  } else {
+    hipLaunchKernelGGL((k_tree_bfs<CAP, true>), dim3(d.nh), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree<CAP, true>), dim3(d.nh), dim3(64), 0, s, d);
+  }
+}
+
 void launch_tree(const Dev &d, int pass, hipStream_t s) {
   if (d.nh == 0) return;
-  if (d.cap <= 64) {
-    if (pass == 0) hipLaunchKernelGGL((k_tree<64, false>), dim3(d.nh), dim3(64), 0, s, d);
-    else hipLaunchKernelGGL((k_tree<64, true>), dim3(d.nh), dim3(64), 0, s, d);
-  } else if (d.cap <= 128) {
-    if (pass == 0) hipLaunchKernelGGL((k_tree<128, false>), dim3(d.nh), dim3(64), 0, s, d);
-    else hipLaunchKernelGGL((k_tree<128, true>), dim3(d.nh), dim3(64), 0, s, d);
-  } else {
-    if (pass == 0) hipLaunchKernelGGL((k_tree<256, false>), dim3(d.nh), dim3(64), 0, s, d);
-    else hipLaunchKernelGGL((k_tree<256, true>), dim3(d.nh), dim3(64), 0, s, d);
-  }
+  if (d.cap <= 64) launch_tree_cap<64>(d, pass, s);
+  else if (d.cap <= 128) launch_tree_cap<128>(d, pass, s);
+  else launch_tree_cap<256>(d, pass, s);
 }
 
 }  // namespace agb
