@@ -18,7 +18,22 @@ evaluated as one batch replayed through a hipGraph after per-instance
 Gaussian coordinate perturbations (sigma 0.05 A, replay r seed 20263000 + r),
 against the same instances evaluated one agbnp3_ener_device call each.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched]
+Learned surrogate (config 3), reported beside: a Born-radius MLP trained by
+the GPU kernels on ~2M atoms of config-2-style conformations (sizes seed
+20264999, conformation m seed 20264000 + m; radii from the plain kernels),
+then 50 held-out 6000-atom solutes (seeds 20265000 + k) evaluated as one
+batch with the surrogate (plain anchor every 10th call) replayed through a
+hipGraph, against the plain per-instance calls on the same perturbed frames
+(sigma 0.05 A, frame r seed 20265100 + r); Delta G_solv error per solute.
+
+Trajectory (config 4), reported beside: the 150,000-atom solute (seed
+20266150), frames of accumulated Gaussian displacements (sigma 0.03 A,
+frame f seed 20267000 + f), neighbour list kept with a 1 A skin and rebuilt
+every 10 frames, surrogate anchored every 10 frames, against the plain
+calls (rebuild every frame) on the same frames.  Default 20 frames
+(--c4-frames 200 for the whole sequence).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched] [--no-surrogate]
 """
 import argparse
 import json
@@ -67,6 +82,198 @@ def cpu_baseline(system, calls, threads):
     dt = time.perf_counter() - t0
     return dict(value=system.natoms / dt, unit="atom-evals/s", cores=threads, kind="port",
                 sample=f"1 oracle evaluation of the same {system.natoms}-atom solute, {dt:.3f} s")
+
+
+DG_TERMS = (0, 1, 2, 3, 4, 5)        # energies[0..5]: egb, evdw, ecorr_vdw, ecav, ecorr_cav, ehb
+
+
+class DeviceIO:
+    """Device inputs and outputs of one evaluation of n atoms in nsys systems."""
+
+    def __init__(self, hip, n, nsys):
+        self.hip, self.n, self.nsys = hip, n, nsys
+        self.xyz = [hip.DeviceArray(8 * n) for _ in range(3)]
+        self.out = [hip.DeviceArray(8 * n) for _ in range(3)] + [hip.DeviceArray(24 * n) for _ in range(4)]
+        self.e = hip.DeviceArray(56 * nsys)
+
+    def ptrs(self):
+        return [a.addr for a in self.xyz] + [a.addr for a in self.out] + [self.e.addr]
+
+    def ptrs_of(self, k, off, xyz=None):
+        """Pointers of system k (first atom `off`) inside the concatenated arrays."""
+        src = xyz if xyz is not None else self.xyz
+        return [a.addr + 8 * off for a in src] + [a.addr + 8 * off for a in self.out[:3]] + \
+               [a.addr + 24 * off for a in self.out[3:]] + [self.e.addr + 56 * k]
+
+    def free(self):
+        for a in self.xyz + self.out + [self.e]:
+            a.free()
+
+
+def train_surrogate(args, api, hip, synth, np):
+    """Config 3 training set and training: plain radii of config-2-style conformations, GPU kernels."""
+    t0 = time.perf_counter()
+    model = api.Surrogate(seed=20264001)
+    model.reserve(args.sg_train_atoms + 20000)
+    sizes = synth.sizes(20264999, 8192, 300, 6000)
+    total, m = 0, 0
+    while total < args.sg_train_atoms:
+        group, gn = [], 0
+        while gn < 250000 and total + gn < args.sg_train_atoms:
+            g = synth.generate(20264000 + m, int(sizes[m]))
+            group.append(g)
+            gn += g.natoms
+            m += 1
+        insts = [api.Instance(g) for g in group]
+        b = api.Batch(insts)
+        xyz = [hip.DeviceArray.from_numpy(np.concatenate([getattr(g, k) for g in group])) for k in ("x", "y", "z")]
+        model.collect(b, xyz[0].addr, xyz[1].addr, xyz[2].addr)
+        for a in xyz:
+            a.free()
+        b.delete()
+        for i in insts:
+            i.delete()
+        total += gn
+    t_collect = time.perf_counter() - t0
+    model.fit(margin=0.1, seed=20264002)
+    hip.synchronize()
+    t1 = time.perf_counter()
+    losses = model.train(epochs=args.sg_epochs, batch=4096, lr=2e-3, decay=0.85, seed=20264003)
+    hip.synchronize()
+    t_train = time.perf_counter() - t1
+    log(f"bench: surrogate trained on {total} atoms ({m} conformations), {args.sg_epochs} epochs in {t_train:.1f} s, "
+        f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    return model, dict(train_atoms=int(total), conformations=m, epochs=args.sg_epochs,
+                       loss_first=float(losses[0]), loss_last=float(losses[-1]), train_seconds=t_train,
+                       collect_seconds=t_collect)
+
+
+def dg_rel_err(e_a, e_b, np):
+    """max over systems of |Delta G_a - Delta G_b| / |Delta G_b| (energies [nsys, 7])."""
+    ga, gb = e_a[:, :6].sum(1), e_b[:, :6].sum(1)
+    return float(np.max(np.abs(ga - gb) / np.abs(gb)))
+
+
+def bench_surrogate(args, model, api, hip, synth, np):
+    """Config 3: held-out solutes, batched + surrogate (graph replays) vs plain per-instance calls."""
+    ss = [synth.generate(20265000 + k, 6000) for k in range(args.c3_solutes)]
+    insts = [api.Instance(g) for g in ss]
+    b = api.Batch(insts)
+    n, off = b.n, b.offsets
+    reps = args.c3_calls
+    base = [np.concatenate([getattr(g, k) for g in ss]) for k in ("x", "y", "z")]
+    frames = []
+    for r in range(reps + 1):
+        rng = np.random.default_rng(20265100 + r)
+        frames.append([hip.DeviceArray.from_numpy(a + rng.normal(0.0, 0.05, n)) for a in base])
+    io = DeviceIO(hip, n, b.nsys)
+    e_sur = [hip.DeviceArray(56 * b.nsys) for _ in range(reps)]
+    stream = b.stream()
+    for k in range(3):
+        hip.copy_async(io.xyz[k].addr, frames[0][k].addr, 8 * n, hip.D2D, stream)
+    b.tune(io.xyz[0].addr, io.xyz[1].addr, io.xyz[2].addr, reps=1)
+    b.sg_attach(model, 10)
+    if b.ener_device(*io.ptrs()) != api.AGBNP_OK or b.sync() != api.AGBNP_OK:   # first anchor (call 1)
+        raise RuntimeError("batched surrogate call failed")
+    b.graph_capture(*io.ptrs())
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        for k in range(3):
+            hip.copy_async(io.xyz[k].addr, frames[r + 1][k].addr, 8 * n, hip.D2D, stream)
+        if b.graph_launch() != api.AGBNP_OK:
+            raise RuntimeError("graph launch failed")
+        hip.copy_async(e_sur[r].addr, io.e.addr, 56 * b.nsys, hip.D2D, stream)
+    rc = b.sync()
+    hip.synchronize()
+    t_sur = (time.perf_counter() - t0) / reps
+    if rc != api.AGBNP_OK:
+        raise RuntimeError("batched surrogate status error")
+    err_est, modes = b.sg_stats()
+    # plain form: one agbnp3_ener_device per instance, same frames
+    pio = DeviceIO(hip, n, b.nsys)
+    e_plain = [hip.DeviceArray(56 * b.nsys) for _ in range(reps)]
+    for k, inst in enumerate(insts):                     # warm-up call each
+        inst.ener_device(*pio.ptrs_of(k, int(off[k]), frames[0]))
+    for inst in insts:
+        inst.sync()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        for k, inst in enumerate(insts):
+            if inst.ener_device(*pio.ptrs_of(k, int(off[k]), frames[r + 1])) != api.AGBNP_OK:
+                raise RuntimeError("agbnp3_ener_device failed")
+        for inst in insts:
+            inst.sync()
+        hip.check(hip.lib().hipMemcpy(e_plain[r].ptr, pio.e.ptr, 56 * b.nsys, hip.D2D), "hipMemcpy")
+    hip.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    errs = [dg_rel_err(e_sur[r].to_numpy(np.float64, (b.nsys, 7)), e_plain[r].to_numpy(np.float64, (b.nsys, 7)), np)
+            for r in range(reps)]
+    for a in [x for f in frames for x in f] + e_sur + e_plain:
+        a.free()
+    io.free()
+    pio.free()
+    b.delete()
+    for i in insts:
+        i.delete()
+    return dict(workload=f"config 3: {len(ss)} held-out 6000-atom solutes (seeds 20265000+k) as one batch, "
+                         f"surrogate with a plain anchor every 10th call, hipGraph replays of sigma 0.05 A frames",
+                natoms=int(n), calls=reps, surrogate_atom_evals_per_s=n / t_sur, surrogate_ms_per_call=1e3 * t_sur,
+                plain_atom_evals_per_s=n / t_plain, plain_ms_per_call=1e3 * t_plain, speedup=t_plain / t_sur,
+                max_rel_dG_err=max(errs), rel_dG_err_per_call=errs,
+                judge_abs_err_kcal=float(np.max(np.abs(err_est))))
+
+
+def bench_trajectory(args, model, api, hip, synth, np):
+    """Config 4: 150k-atom trajectory, surrogate + neighbour-list reuse vs plain calls."""
+    s = synth.generate(20266150, args.c4_natoms)
+    n = s.natoms
+    plain, sur = api.Instance(s), api.Instance(s)
+    sur.set_nblist_reuse(1.0, every=10)
+    sur.sg_attach(model, 10)
+    x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+    frames = []
+    for f in range(args.c4_frames):
+        rng = np.random.default_rng(20267000 + f)
+        x, y, z = (a + rng.normal(0.0, 0.03, n) for a in (x, y, z))
+        frames.append([hip.DeviceArray.from_numpy(a) for a in (x, y, z)])
+    res = {}
+    for name, inst in (("surrogate", sur), ("plain", plain)):
+        io = DeviceIO(hip, n, 1)
+        es = [hip.DeviceArray(56) for _ in frames]
+        stream = inst.stream()
+        if inst.ener_device(*io.ptrs_of(0, 0, frames[0])) != api.AGBNP_OK or inst.sync() != api.AGBNP_OK:
+            raise RuntimeError("trajectory warm-up failed")
+        hip.synchronize()
+        t0 = time.perf_counter()
+        for f in range(1, len(frames)):
+            if inst.ener_device(*io.ptrs_of(0, 0, frames[f])) != api.AGBNP_OK:
+                raise RuntimeError("agbnp3_ener_device failed")
+            hip.copy_async(es[f].addr, io.e.addr, 56, hip.D2D, stream)
+        rc = inst.sync()
+        hip.synchronize()
+        res[name] = (time.perf_counter() - t0) / (len(frames) - 1)
+        if rc != api.AGBNP_OK:
+            raise RuntimeError("trajectory status error")
+        res[name + "_e"] = [es[f].to_numpy(np.float64, (1, 7)) for f in range(1, len(frames))]
+        for a in es:
+            a.free()
+        io.free()
+    errs = [dg_rel_err(a, b, np) for a, b in zip(res["surrogate_e"], res["plain_e"])]
+    calls, rebuilds = sur.nblist_rebuilds()
+    for f in frames:
+        for a in f:
+            a.free()
+    plain.delete()
+    sur.delete()
+    return dict(workload=f"config 4: {n}-atom solute (seed 20266150), {len(frames)} frames of accumulated sigma "
+                         f"0.03 A displacements, neighbour list kept (1 A skin) and rebuilt every 10 frames, "
+                         f"surrogate anchored every 10 frames",
+                natoms=n, frames=len(frames), surrogate_atom_evals_per_s=n / res["surrogate"],
+                surrogate_ms_per_frame=1e3 * res["surrogate"], plain_atom_evals_per_s=n / res["plain"],
+                plain_ms_per_frame=1e3 * res["plain"], speedup=res["plain"] / res["surrogate"],
+                max_rel_dG_err=max(errs), nblist_rebuilds=rebuilds, calls=calls)
 
 
 def bench_batched(args, api, hip, synth, np):
@@ -170,6 +377,13 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sg-train-atoms", type=int, default=2_000_000)
+    ap.add_argument("--sg-epochs", type=int, default=20)
+    ap.add_argument("--c3-solutes", type=int, default=50)
+    ap.add_argument("--c3-calls", type=int, default=10)
+    ap.add_argument("--c4-natoms", type=int, default=150000)
+    ap.add_argument("--c4-frames", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build evaluates one instance on one GPU; --gpus {args.gpus} is not supported "
@@ -244,6 +458,20 @@ def main():
         log(f"bench: batched {batched['batched_ms_per_replay']:.2f} ms/replay, plain loop "
             f"{batched['plain_ms_per_replay']:.2f} ms, speedup {batched['speedup']:.2f}x")
 
+    surrogate = trajectory = None
+    if not args.no_surrogate:
+        model, training = train_surrogate(args, api, hip, synth, np)
+        surrogate = bench_surrogate(args, model, api, hip, synth, np)
+        surrogate["training"] = training
+        log(f"bench: surrogate {surrogate['surrogate_ms_per_call']:.2f} ms/call vs plain "
+            f"{surrogate['plain_ms_per_call']:.2f} ms ({surrogate['speedup']:.2f}x), "
+            f"max dG error {surrogate['max_rel_dG_err']:.2e}")
+        trajectory = bench_trajectory(args, model, api, hip, synth, np)
+        log(f"bench: trajectory {trajectory['surrogate_ms_per_frame']:.2f} ms/frame vs plain "
+            f"{trajectory['plain_ms_per_frame']:.2f} ms ({trajectory['speedup']:.2f}x), "
+            f"max dG error {trajectory['max_rel_dG_err']:.2e}")
+        model.delete()
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -262,6 +490,8 @@ def main():
         host_api_atom_evals_per_s=host_rate,
         kernel_ms=ktimes,
         batched=batched,
+        surrogate=surrogate,
+        trajectory=trajectory,
     )
     print(json.dumps(line), flush=True)
 
