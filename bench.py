@@ -384,6 +384,8 @@ def main():
     ap.add_argument("--c3-calls", type=int, default=10)
     ap.add_argument("--c4-natoms", type=int, default=150000)
     ap.add_argument("--c4-frames", type=int, default=20)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (counter calibration in profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build evaluates one instance on one GPU; --gpus {args.gpus} is not supported "
@@ -436,6 +438,14 @@ def main():
     same = all(e_dev[i] == host_out[k] for i, k in enumerate(api.ENERGY_NAMES))
     log(f"bench: {ms_per_step:.3f} ms/step, {value:.4g} atom-evals/s; device outputs equal host-API call: {same}")
     log("bench: kernel ms " + ", ".join(f"{k}={v:.3f}" for k, v in ktimes.items()))
+
+    if args.copy_calibration_mb:
+        nb = args.copy_calibration_mb << 20
+        src, dst = hip.DeviceArray(nb), hip.DeviceArray(nb)
+        hip.copy_async(dst.addr, src.addr, nb, hip.D2D, None)
+        hip.synchronize()
+        src.free()
+        dst.free()
 
     # PCIe-inclusive host API rate (reported beside, never as value)
     th0 = time.perf_counter()
