@@ -297,10 +297,11 @@ class Instance:
         return out
 
     def counts(self):
-        c = np.zeros(5, np.int64)
+        c = np.zeros(7, np.int64)
         load().agbnp3_get_counts(self.tag, _p(c))
         return dict(kept_overlaps=int(c[0]), evaluated_overlaps=int(c[1]), water_sites=int(c[2]),
-                    near_pairs=int(c[3]), max_row=int(c[4]))
+                    near_pairs=int(c[3]), max_row=int(c[4]), tree_fallback_volumes=int(c[5]),
+                    tree_fallback_derivatives=int(c[6]))
 
     def kernel_times(self):
         ms = np.zeros(64, np.float32)

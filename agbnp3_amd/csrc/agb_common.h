@@ -127,7 +127,7 @@ struct SgTrain {
 };
 
 // work-queue heads (cleared by the ingest kernel of every call)
-enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_COUNT };
+enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_TREE_A, Q_TREE_B, Q_TREE_FA, Q_TREE_FB, Q_COUNT };
 
 // Device view of a pack of one or more systems, passed by value to every kernel.
 struct Dev {

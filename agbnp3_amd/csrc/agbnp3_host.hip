@@ -1329,7 +1329,7 @@ int agbnp3_get_internal(int tag, const char *name, double *out) {
 int agbnp3_get_counts(int tag, long long *counts) {
   Instance *I = get_instance(tag, "agbnp3_get_counts()");
   if (!I) return AGBNP_ERR;
-  unsigned long long c[4];
+  unsigned long long c[6];
   HIPCHK(hipStreamSynchronize(I->p.stream));
   HIPCHK(hipMemcpy(c, I->p.d.counters, sizeof(c), hipMemcpyDeviceToHost));
   counts[0] = (long long)c[0];
@@ -1337,6 +1337,8 @@ int agbnp3_get_counts(int tag, long long *counts) {
   counts[2] = I->m.nws;
   counts[3] = (long long)c[2];
   counts[4] = (long long)c[3];
+  counts[5] = (long long)c[4];
+  counts[6] = (long long)c[5];
   return AGBNP_OK;
 }
 

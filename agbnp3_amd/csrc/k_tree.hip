@@ -300,11 +300,8 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
 // depth-first walk of the roots the breadth-first kernel queued (overflow);
 // the grid covers every heavy row, blocks past the queue length exit
 template <int CAP, bool PB>
-__global__ __launch_bounds__(64) void k_tree(Dev d) {
-  __shared__ TreeLds<CAP, PB> S;
+__device__ void tree_dfs_root(const Dev &d, TreeLds<CAP, PB> &S, int h) {
   const int lane = threadIdx.x;
-  if (blockIdx.x >= (unsigned)d.counters[4 + (PB ? 1 : 0)]) return;
-  const int h = d.tree_ovf[(PB ? d.nh : 0) + blockIdx.x];
   const int i = d.hrow[h];
   const int m = d.nne[h];
   // stage the slot atoms
@@ -370,6 +367,18 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
     atomicAdd(&d.counters[0], (unsigned long long)nkept);
     atomicAdd(&d.counters[1], (unsigned long long)neval);
   }
+  __syncthreads();
+}
+
+// the depth-first walk of the queued roots, persistent blocks
+template <int CAP, bool PB>
+__global__ __launch_bounds__(64) void k_tree(Dev d) {
+  __shared__ TreeLds<CAP, PB> S;
+  __shared__ int slot;
+  const int count = (int)d.counters[4 + (PB ? 1 : 0)];
+  unsigned *head = &d.queue[PB ? Q_TREE_FB : Q_TREE_FA];
+  for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
+    tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(PB ? d.nh : 0) + k]);
 }
 
 // ------------------------------------------------------ breadth-first tree
@@ -479,11 +488,11 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB> &S, int cb, int n, doub
   return nn;
 }
 
+// one root of the breadth-first tree (the whole wave); results scattered to
+// the atoms, or the root queued for the depth-first kernel on overflow
 template <int CAP, bool PB>
-__global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
-  __shared__ BfsLds<CAP, PB> S;
+__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB> &S, int h) {
   const int lane = threadIdx.x;
-  const int h = blockIdx.x;
   const int i = d.hrow[h];
   const int m = d.nne[h];
   for (int s = lane; s <= m; s += 64) {
@@ -553,6 +562,7 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
       const unsigned k = atomicAdd(&d.counters[4 + (PB ? 1 : 0)], 1ull);
       d.tree_ovf[(PB ? d.nh : 0) + k] = h;
     }
+    __syncthreads();
     return;
   }
 #pragma unroll
@@ -577,14 +587,28 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   }
 }
 
+
+// persistent: each block takes roots from a queue until none are left
+template <int CAP, bool PB>
+__global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
+  __shared__ BfsLds<CAP, PB> S;
+  __shared__ int slot;
+  unsigned *head = &d.queue[PB ? Q_TREE_B : Q_TREE_A];
+  for (int h = next_item(head, &slot); h < d.nh; h = next_item(head, &slot))
+    tree_bfs_root<CAP, PB>(d, S, h);
+}
+
+// persistent grids: resident blocks for the breadth-first pass, a small
+// grid for the (usually empty) depth-first queue
 template <int CAP>
 static void launch_tree_cap(const Dev &d, int pass, hipStream_t s) {
+  const int g = std::min(d.nh, 256 * 16), gf = std::min(d.nh, 256);
   if (pass == 0) {
-    hipLaunchKernelGGL((k_tree_bfs<CAP, false>), dim3(d.nh), dim3(64), 0, s, d);
-    hipLaunchKernelGGL((k_tree<CAP, false>), dim3(d.nh), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree_bfs<CAP, false>), dim3(g), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree<CAP, false>), dim3(gf), dim3(64), 0, s, d);
   } else {
-    hipLaunchKernelGGL((k_tree_bfs<CAP, true>), dim3(d.nh), dim3(64), 0, s, d);
-    hipLaunchKernelGGL((k_tree<CAP, true>), dim3(d.nh), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree_bfs<CAP, true>), dim3(g), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree<CAP, true>), dim3(gf), dim3(64), 0, s, d);
   }
 }
 

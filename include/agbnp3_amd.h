@@ -164,7 +164,8 @@ int agbnp3_get_tables(int tag, float *y, float *y2, float *radii);
    "swf", "A", "q2ab", "abrw", "deru", "derv", "derus", "dervs", "derh" */
 int agbnp3_get_internal(int tag, const char *name, double *out);
 /* counts[0] kept overlaps, [1] evaluated overlaps, [2] water sites,
-   [3] neighbour-list entries, [4] max row length */
+   [3] neighbour-list entries, [4] max row length, [5] [6] overlap-tree roots
+   that outgrew the breadth-first level buffers (volumes, derivatives) */
 int agbnp3_get_counts(int tag, long long *counts);
 /* per-kernel times (ms), averaged over the calls made since the previous
    read while AGBNP3_OPT_TIMING was set (HIP events on the instance stream);
