@@ -127,7 +127,7 @@ struct SgTrain {
 };
 
 // work-queue heads (cleared by the ingest kernel of every call)
-enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_TREE_A, Q_TREE_B, Q_TREE_FA, Q_TREE_FB, Q_COUNT };
+enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_TREE_A, Q_TREE_B, Q_TREE_BA, Q_TREE_BB, Q_TREE_FA, Q_TREE_FB, Q_COUNT };
 
 // Device view of a pack of one or more systems, passed by value to every kernel.
 struct Dev {
@@ -189,8 +189,9 @@ struct Dev {
   long long *acc_gbA, *acc_gbg;        // GB pairs: A_i [n], gradient [n][3]
   long long *acc_ch;                   // chain rule [n][8]: gb xyz, vdw xyz, U, V
   unsigned long long *counters;        // [0] kept overlaps [1] evaluated [2] nbl entries [3] max row
-                                       // [4], [5] roots queued for the depth-first tree (pass A, B)
-  int *tree_ovf;                       // [2][nh] those roots' heavy rows
+                                       // [4], [5] roots queued for the second breadth-first tier (pass A, B)
+                                       // [6], [7] roots queued for the depth-first walk
+  int *tree_ovf;                       // [4][nh] those roots' heavy rows
   double *volumep_raw, *sa_raw, *sa_f, *sp, *spe, *psvol, *gammap;   // [n]
   float *sp_f, *spe_f;                 // FP32 copies for pair kernels
   double *beta, *br, *brw, *swf;       // [n]
@@ -234,6 +235,7 @@ void launch_ingest(const Dev &d, const double *x, const double *y, const double 
 void launch_cells(const Dev &d, hipStream_t s);
 void launch_nblist(const Dev &d, hipStream_t s);
 void launch_tree(const Dev &d, int pass, hipStream_t s);
+hipError_t prepare_tree();
 void launch_scale(const Dev &d, hipStream_t s);
 void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s);
 void launch_born(const Dev &d, hipStream_t s);

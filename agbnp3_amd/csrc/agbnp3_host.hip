@@ -596,8 +596,8 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
       dalloc(L, &d.cell_tmp, nn) || dalloc(L, &d.nne, nhh) || dalloc(L, &d.nbr, nhh * 256) ||
       dalloc(L, &d.cand_nne, nhh) || dalloc(L, &d.cand_nbr, nhh * kCandCap) || dalloc(L, &d.ref_pos, nn) ||
       dalloc(L, &d.nbl_state, 4) ||
-      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, Q_COUNT) || dalloc(L, &d.counters, 6) ||
-      dalloc(L, &d.tree_ovf, 2 * nhh) ||
+      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, Q_COUNT) || dalloc(L, &d.counters, 8) ||
+      dalloc(L, &d.tree_ovf, 4 * nhh) ||
       dalloc(L, &d.acc_vol, nn) || dalloc(L, &d.acc_sa, nn) || dalloc(L, &d.acc_grad, nn * TG_N) ||
       dalloc(L, &d.acc_hb, 3 * nn) || dalloc(L, &d.acc_derh, nn) || dalloc(L, &d.acc_beta, nn) ||
       dalloc(L, &d.acc_gbA, nn) || dalloc(L, &d.acc_gbg, 3 * nn) || dalloc(L, &d.acc_ch, 8 * nn) ||
@@ -619,6 +619,7 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
   HIPCHK(hipHostMalloc((void **)&P.hstage, (18 * nn + 7 * (size_t)P.nsys) * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   HIPCHK(prepare_pairs(P.ntypes));
+  HIPCHK(prepare_tree());
   return build_items(P);
 }
 
@@ -1329,7 +1330,7 @@ int agbnp3_get_internal(int tag, const char *name, double *out) {
 int agbnp3_get_counts(int tag, long long *counts) {
   Instance *I = get_instance(tag, "agbnp3_get_counts()");
   if (!I) return AGBNP_ERR;
-  unsigned long long c[6];
+  unsigned long long c[8];
   HIPCHK(hipStreamSynchronize(I->p.stream));
   HIPCHK(hipMemcpy(c, I->p.d.counters, sizeof(c), hipMemcpyDeviceToHost));
   counts[0] = (long long)c[0];
@@ -1337,8 +1338,8 @@ int agbnp3_get_counts(int tag, long long *counts) {
   counts[2] = I->m.nws;
   counts[3] = (long long)c[2];
   counts[4] = (long long)c[3];
-  counts[5] = (long long)c[4];
-  counts[6] = (long long)c[5];
+  counts[5] = (long long)(c[4] + c[6]);
+  counts[6] = (long long)(c[5] + c[7]);
   return AGBNP_OK;
 }
 

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
   }
   if (i == 0) {
     *d.status = 0;
-    for (int k = 0; k < 6; k++) d.counters[k] = 0;
+    for (int k = 0; k < 8; k++) d.counters[k] = 0;
     for (int k = 0; k < Q_COUNT; k++) d.queue[k] = 0;
     if (d.skin > 0.f) {       // neighbour-list reuse: forced rebuilds
       const int calls = d.nbl_state[1] + 1;

@@ -370,21 +370,22 @@ __device__ void tree_dfs_root(const Dev &d, TreeLds<CAP, PB> &S, int h) {
   __syncthreads();
 }
 
-// the depth-first walk of the queued roots, persistent blocks
+// the depth-first walk of the roots both breadth-first tiers queued
 template <int CAP, bool PB>
 __global__ __launch_bounds__(64) void k_tree(Dev d) {
   __shared__ TreeLds<CAP, PB> S;
   __shared__ int slot;
-  const int count = (int)d.counters[4 + (PB ? 1 : 0)];
+  const int count = (int)d.counters[6 + (PB ? 1 : 0)];
   unsigned *head = &d.queue[PB ? Q_TREE_FB : Q_TREE_FA];
   for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
-    tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(PB ? d.nh : 0) + k]);
+    tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(size_t)(2 + (PB ? 1 : 0)) * d.nh + k]);
 }
 
 // ------------------------------------------------------ breadth-first tree
 constexpr int kLevelCap = 256;          // nodes of one order held in LDS
+constexpr int kLevelCapBig = 1536;      // second tier (dynamic LDS) for the largest trees
 
-template <int CAP, bool PB>
+template <int CAP, bool PB, int LC>
 struct BfsLds {
   int idx[CAP + 1];
   float a[CAP + 1], x[CAP + 1], y[CAP + 1], z[CAP + 1], R[CAP + 1];
@@ -393,11 +394,10 @@ struct BfsLds {
   double acc[PB ? TG_N : 2][CAP + 1];
   // two node buffers: Gaussian, member slots (16 bits each, root excluded),
   // end of the node's sibling group, parent index in the previous order
-  float ga[2][kLevelCap], gp[2][kLevelCap], gx[2][kLevelCap], gy[2][kLevelCap], gz[2][kLevelCap];
-  unsigned long long path[2][kLevelCap];
-  int gend[2][kLevelCap], gpar[2][kLevelCap];
-  int pref[kLevelCap + 1], cnt[kLevelCap];
-  int ovf;
+  float ga[2][LC], gp[2][LC], gx[2][LC], gy[2][LC], gz[2][LC];
+  unsigned long long path[2][LC];
+  int gend[2][LC], gpar[2][LC];
+  int pref[LC + 1], cnt[LC];
 };
 
 // exclusive scan of vals[0..n) into out[0..n], out[n] = total (one wave)
@@ -417,9 +417,9 @@ __device__ __forceinline__ int wave_exclusive_scan(const int *vals, int *out, in
 
 // children of order N: pairs (u, later sibling v) of the nodes of order N-1
 // in buffer `cb` (n nodes), kept ones into buffer 1-cb; returns their count
-// (kLevelCap + 1 on overflow)
-template <int N, int CAP, bool PB, bool STORE>
-__device__ __forceinline__ int bfs_order(BfsLds<CAP, PB> &S, int cb, int n, double (&root)[TG_N], float df,
+// (LC + 1 on overflow)
+template <int N, int CAP, bool PB, int LC, bool STORE>
+__device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, double (&root)[TG_N], float df,
                                          unsigned &nkept, unsigned &neval) {
   const int lane = threadIdx.x & 63;
   const int nb = 1 - cb;
@@ -462,7 +462,7 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB> &S, int cb, int n, doub
       add_contrib<N, CAP, PB>(S, o, mem, root, df);
       if (STORE) {
         const int pos = nn + __popcll(bal & ((1ull << lane) - 1ull));
-        if (pos < kLevelCap) {
+        if (pos < LC) {
           S.ga[nb][pos] = o.a; S.gp[nb][pos] = o.p;
           S.gx[nb][pos] = o.cx; S.gy[nb][pos] = o.cy; S.gz[nb][pos] = o.cz;
           S.path[nb][pos] = pu | ((unsigned long long)sb << (16 * (N - 2)));
@@ -476,7 +476,7 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB> &S, int cb, int n, doub
   nkept += (unsigned)nn;
   __syncthreads();
   if (!STORE) return 0;
-  if (nn > kLevelCap) return kLevelCap + 1;
+  if (nn > LC) return LC + 1;
   // a node's siblings are the kept children of its parent, contiguous
   wave_exclusive_scan(S.cnt, S.pref, n);
   __syncthreads();
@@ -490,8 +490,8 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB> &S, int cb, int n, doub
 
 // one root of the breadth-first tree (the whole wave); results scattered to
 // the atoms, or the root queued for the depth-first kernel on overflow
-template <int CAP, bool PB>
-__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB> &S, int h) {
+template <int CAP, bool PB, int LC>
+__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int list) {
   const int lane = threadIdx.x;
   const int i = d.hrow[h];
   const int m = d.nne[h];
@@ -535,7 +535,7 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB> &S, int h) {
         const int mem[2] = {0, sb};
         add_contrib<2, CAP, PB>(S, o, mem, root, df);
         const int pos = n + __popcll(bal & ((1ull << lane) - 1ull));
-        if (pos < kLevelCap) {
+        if (pos < LC) {
           S.ga[0][pos] = o.a; S.gp[0][pos] = o.p;
           S.gx[0][pos] = o.cx; S.gy[0][pos] = o.cy; S.gz[0][pos] = o.cz;
           S.path[0][pos] = (unsigned long long)sb;
@@ -544,23 +544,24 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB> &S, int h) {
       n += __popcll(bal);
     }
     nkept += (unsigned)n;
-    if (n > kLevelCap) ovf = true;
-    for (int k = lane; k < n && k < kLevelCap; k += 64) S.gend[0][k] = n;
+    if (n > LC) ovf = true;
+    for (int k = lane; k < n && k < LC; k += 64) S.gend[0][k] = n;
     __syncthreads();
   }
   if (!ovf && n > 1) {
-    n = bfs_order<3, CAP, PB, true>(S, 0, n, root, df, nkept, neval);
-    if (n > kLevelCap) ovf = true;
+    n = bfs_order<3, CAP, PB, LC, true>(S, 0, n, root, df, nkept, neval);
+    if (n > LC) ovf = true;
     else if (n > 1) {
-      n = bfs_order<4, CAP, PB, true>(S, 1, n, root, df, nkept, neval);
-      if (n > kLevelCap) ovf = true;
-      else if (n > 1) bfs_order<5, CAP, PB, false>(S, 0, n, root, df, nkept, neval);
+      n = bfs_order<4, CAP, PB, LC, true>(S, 1, n, root, df, nkept, neval);
+      if (n > LC) ovf = true;
+      else if (n > 1) bfs_order<5, CAP, PB, LC, false>(S, 0, n, root, df, nkept, neval);
     }
   }
-  if (ovf) {                              // wave-uniform: the depth-first kernel takes this root
+  if (ovf) {                              // wave-uniform: the next tier takes this root
     if (lane == 0) {
-      const unsigned k = atomicAdd(&d.counters[4 + (PB ? 1 : 0)], 1ull);
-      d.tree_ovf[(PB ? d.nh : 0) + k] = h;
+      const int q = 4 + 2 * list + (PB ? 1 : 0);
+      const unsigned k = atomicAdd(&d.counters[q], 1ull);
+      d.tree_ovf[(size_t)(q - 4) * d.nh + k] = h;
     }
     __syncthreads();
     return;
@@ -588,26 +589,60 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB> &S, int h) {
 }
 
 
-// persistent: each block takes roots from a queue until none are left
+// persistent: each block takes roots from a queue until none are left;
+// overflowing roots go to the second tier's list
 template <int CAP, bool PB>
 __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
-  __shared__ BfsLds<CAP, PB> S;
+  __shared__ BfsLds<CAP, PB, kLevelCap> S;
   __shared__ int slot;
   unsigned *head = &d.queue[PB ? Q_TREE_B : Q_TREE_A];
   for (int h = next_item(head, &slot); h < d.nh; h = next_item(head, &slot))
-    tree_bfs_root<CAP, PB>(d, S, h);
+    tree_bfs_root<CAP, PB, kLevelCap>(d, S, h, 0);
 }
 
-// persistent grids: resident blocks for the breadth-first pass, a small
-// grid for the (usually empty) depth-first queue
+// second tier: the roots the first tier queued, with 1536-node levels in
+// dynamic LDS; what still overflows goes to the depth-first kernel
+template <int CAP, bool PB>
+__global__ __launch_bounds__(64) void k_tree_bfs_big(Dev d) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  BfsLds<CAP, PB, kLevelCapBig> &S = *reinterpret_cast<BfsLds<CAP, PB, kLevelCapBig> *>(lds_raw);
+  __shared__ int slot;
+  const int count = (int)d.counters[4 + (PB ? 1 : 0)];
+  unsigned *head = &d.queue[PB ? Q_TREE_BB : Q_TREE_BA];
+  for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
+    tree_bfs_root<CAP, PB, kLevelCapBig>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1);
+}
+
+template <int CAP, bool PB>
+static size_t big_lds() { return sizeof(BfsLds<CAP, PB, kLevelCapBig>); }
+
+hipError_t prepare_tree() {
+  hipError_t e = hipSuccess;
+  auto set = [&](const void *f, size_t bytes) {
+    if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  };
+  set((const void *)k_tree_bfs_big<64, false>, big_lds<64, false>());
+  set((const void *)k_tree_bfs_big<64, true>, big_lds<64, true>());
+  set((const void *)k_tree_bfs_big<128, false>, big_lds<128, false>());
+  set((const void *)k_tree_bfs_big<128, true>, big_lds<128, true>());
+  set((const void *)k_tree_bfs_big<256, false>, big_lds<256, false>());
+  set((const void *)k_tree_bfs_big<256, true>, big_lds<256, true>());
+  return e;
+}
+
+// persistent grids: resident blocks for the breadth-first pass, small grids
+// for the (short) second-tier and depth-first queues
 template <int CAP>
 static void launch_tree_cap(const Dev &d, int pass, hipStream_t s) {
   const int g = std::min(d.nh, 256 * 16), gf = std::min(d.nh, 256);
+  const size_t la = big_lds<CAP, false>(), lb = big_lds<CAP, true>();
   if (pass == 0) {
     hipLaunchKernelGGL((k_tree_bfs<CAP, false>), dim3(g), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree_bfs_big<CAP, false>), dim3(gf), dim3(64), la, s, d);
     hipLaunchKernelGGL((k_tree<CAP, false>), dim3(gf), dim3(64), 0, s, d);
   } else {
     hipLaunchKernelGGL((k_tree_bfs<CAP, true>), dim3(g), dim3(64), 0, s, d);
+    hipLaunchKernelGGL((k_tree_bfs_big<CAP, true>), dim3(gf), dim3(64), lb, s, d);
     hipLaunchKernelGGL((k_tree<CAP, true>), dim3(gf), dim3(64), 0, s, d);
   }
 }
