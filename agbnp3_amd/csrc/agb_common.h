@@ -158,7 +158,10 @@ struct Dev {
   // work items, sorted by decreasing size, taken from the queues by
   // persistent blocks: descreening / chain (row block, column range), GB
   // tile pairs (row range, column range; diagonal when they start together)
-  const int4 *desc_items, *chain_items, *gb_items;
+  const int4 *desc_items, *chain_items, *gb_items;   // chain items: (block P, block Q) of cblk
+  const int *cperm;      // chain-rule blocks: [ncblk][256] pack atom or -1
+  const int4 *cblk;      // per chain block: radius type, system, hydrogen rows only, -
+  int ncblk;
   int ndesc_items, nchain_items, ngb_items;
   const int *gb_eslot;   // energy slot of each GB item
   unsigned *queue;       // [Q_COUNT]
@@ -224,7 +227,7 @@ struct Dev {
 struct Tuning {
   int rows = 256;          // rows per spline-kernel block
   int desc_chunk = 0;      // columns per descreening item (0: whole system)
-  int chain_chunk = 0;     // columns per chain item (0: whole system)
+  int chain_chunk = 0;     // reserved (the chain rule works on 256 x 256 tile pairs)
   int desc_grid = 0;       // persistent blocks (0: one per item)
   int gb_grid = 0;
   int chain_grid = 0;

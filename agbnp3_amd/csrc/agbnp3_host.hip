@@ -464,7 +464,44 @@ int build_items(Pack &P) {
     return items;
   };
   std::vector<int4> desc = chunked(P.tune.desc_chunk, false);
-  std::vector<int4> chain = chunked(P.tune.chain_chunk, true);
+  // chain rule: pairs (P <= Q) of 256-atom blocks of one radius type, inside
+  // one system, not both hydrogen-only
+  std::vector<int> cperm;
+  std::vector<int4> cblk;
+  std::vector<int> csize;
+  std::vector<std::pair<int, int>> csys;             // first and end block of each system
+  for (int b = 0; b < P.nsys; b++) {
+    const SysInfo &S = P.sinfo[b];
+    const int first = (int)cblk.size();
+    for (int t = 0; t < P.ntypes; t++) {
+      std::vector<int> r;
+      for (int i = S.a0; i < S.a0 + S.n; i++)
+        if (P.rtype[i] == t) r.push_back(i);
+      for (size_t k = 0; k < r.size(); k += 256) {
+        int allh = 1, cnt = 0;
+        for (int q = 0; q < 256; q++) {
+          const int a = k + q < r.size() ? r[k + q] : -1;
+          cperm.push_back(a);
+          if (a >= 0) { cnt++; if (P.heavy[a]) allh = 0; }
+        }
+        cblk.push_back(make_int4(t, b, allh, 0));
+        csize.push_back(cnt);
+      }
+    }
+    csys.push_back({first, (int)cblk.size()});
+  }
+  std::vector<std::pair<int4, long long>> cit;
+  for (const auto &fe : csys)
+    for (int p = fe.first; p < fe.second; p++)
+      for (int q = p; q < fe.second; q++) {
+        if (cblk[p].z && cblk[q].z) continue;
+        cit.push_back({make_int4(p, q, 0, 0), (long long)csize[p] * csize[q]});
+      }
+  std::stable_sort(cit.begin(), cit.end(), [](const std::pair<int4, long long> &a, const std::pair<int4, long long> &b) {
+    return a.second > b.second;
+  });
+  std::vector<int4> chain(cit.size());
+  for (size_t k = 0; k < cit.size(); k++) chain[k] = cit[k].first;
   // GB tile pairs of 256-atom tiles, I <= J
   std::vector<std::pair<int4, int>> gb;
   for (int b = 0; b < P.nsys; b++) {
@@ -489,12 +526,14 @@ int build_items(Pack &P) {
   }
   if (upload(P.item_allocs, &d.row_perm, row_perm) || upload(P.item_allocs, &d.rblk, rblk) ||
       upload(P.item_allocs, &d.desc_items, desc) || upload(P.item_allocs, &d.chain_items, chain) ||
+      upload(P.item_allocs, &d.cperm, cperm) || upload(P.item_allocs, &d.cblk, cblk) ||
       upload(P.item_allocs, &d.gb_items, gbi) || upload(P.item_allocs, &d.gb_eslot, gbs))
     return AGBNP_ERR;
   d.rows = rows;
   d.nrow_blocks = (int)rblk.size();
   d.ndesc_items = (int)desc.size();
   d.nchain_items = (int)chain.size();
+  d.ncblk = (int)cblk.size();
   d.ngb_items = (int)gbi.size();
   return AGBNP_OK;
 }
@@ -900,9 +939,8 @@ int pack_tune(Pack &P, const double *x, const double *y, const double *z, int re
     std::vector<std::string> kernels;
   };
   const std::vector<Axis> axes = {
-      {TUNE_ROWS, {64, 128, 256}, {"descreen", "chain"}},
+      {TUNE_ROWS, {64, 128, 256}, {"descreen"}},
       {TUNE_DESC_CHUNK, {0, 512, 1024, 2048, 4096}, {"descreen"}},
-      {TUNE_CHAIN_CHUNK, {0, 512, 1024, 2048, 4096}, {"chain"}},
       {TUNE_DESC_GRID, {0, 2 * cu, 4 * cu, 8 * cu}, {"descreen"}},
       {TUNE_GB_GRID, {0, 2 * cu, 4 * cu, 8 * cu}, {"gbpair"}},
       {TUNE_CHAIN_GRID, {0, 2 * cu, 4 * cu, 8 * cu}, {"chain"}},

@@ -7,11 +7,11 @@
 // until the list is exhausted, so systems of very different size (a batch)
 // share one launch with balanced load.
 //
-// Spline kernels (descreening, chain rule): an item is a block of rows of one
-// radius type and one system against a range of that system's columns; the
-// block stages the spline tables its rows can use in LDS, streams the columns
-// through LDS 256 at a time and each row (one per thread) sums a column tile
-// in FP32.  GB pairs: an item is a pair of 256-atom tiles of one system.
+// Descreening: an item is a block of rows of one radius type and one system
+// against a range of that system's columns; the block stages the spline
+// tables its rows can use in LDS, streams the columns through LDS 256 at a
+// time and each row (one per thread) sums a column tile in FP32.  GB pairs
+// and the chain rule: an item is a pair of 256-atom tiles of one system.
 // Row sums leave each item through order-independent fixed-point atomics,
 // so results are bitwise reproducible whatever the schedule.
 #include "agb_common.h"
@@ -200,39 +200,56 @@ void launch_gbpair(const Dev &d, const Tuning &t, hipStream_t s) {
 }
 
 // -------------------------------------------- chain rule at constant sp
-// Row k collects both directed descreening pairs with every atom j:
-//   k <- j (j heavy):  q2ab_k dQ4(k<-j) sp_j      (table [t_k][t_j], R_j)
-//   j <- k (k heavy):  q2ab_j dQ4(j<-k) sp_k      (table [t_j][t_k], R_k)
-// h = df/(4pi) (sum)/d along (x_j - x_k) for the GB gradient, the same with
-// abrw and 1/(4pi) for vdW; U_k = sum_j q2ab_j Q4(j<-k), V_k likewise with abrw.
-// (reference agbnp3_gb_ders_constvp_nolist_ps kernels.c:681-805 and
-//  agbnp3_gb_deruv_nolist_ps kernels.c:809-904, without the q4cache)
-template <int ROWS>
-__global__ __launch_bounds__(ROWS) void k_chain(Dev d) {
-  extern __shared__ float2 dyn_tab[];              // [2][ntypes][512]
-  float2 (*tab_kj)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab);                       // [t_k][*]
-  float2 (*tab_jk)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab + (size_t)d.ntypes * kTblN);  // [*][t_k]
-  __shared__ float4 cpos[kTile];                   // x, y, z, 1/R
-  __shared__ float4 cpar[kTile];                   // sp, q2ab, abrw, rtype bits
+// For a pair (k, j) with at least one heavy atom, both directed descreening
+// terms (reference agbnp3_gb_ders_constvp_nolist_ps kernels.c:681-805 and
+// agbnp3_gb_deruv_nolist_ps kernels.c:809-904, without the q4cache):
+//   k <- j (j heavy): Q4 from table [t_k][t_j] at d/R_j; weights q2ab_k sp_j
+//   j <- k (k heavy): Q4 from table [t_j][t_k] at d/R_k; weights q2ab_j sp_k
+// h = (sum of weight x dQ4) / d along (x_j - x_k), GB with q2ab and vdW with
+// abrw (scaled by df/4pi and 1/4pi later); U_k += q2ab_j Q4(j <- k),
+// V_k += abrw_j Q4(j <- k) and likewise U_j, V_j from Q4(k <- j).
+// Items are pairs (P <= Q) of 256-atom blocks of one radius type each, so
+// a block stages only the two tables [t_P][t_Q] and [t_Q][t_P].  Each
+// unordered pair is evaluated once: rows sit in lanes, the 256 columns are
+// read from LDS while their partial sums rotate one lane per step (DPP), as
+// in the GB pair kernel; diagonal blocks see each pair twice and keep only
+// the row side.
+__global__ __launch_bounds__(256) void k_chain(Dev d) {
+  __shared__ float2 tab_rc[kTblN], tab_cr[kTblN];   // row <- column, column <- row
+  __shared__ float4 cpos[256];                      // x, y, z, 1/R
+  __shared__ float4 cpar[256];                      // sp (0: H), q2ab, abrw, heavy
+  __shared__ float colacc[4][8][256];               // [wave][gb xyz, vw xyz, U, V][column]
   __shared__ int slot;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
   const float sgn2 = (d.options & 2) ? -1.f : 1.f;
-  int staged = -1;
   for (int it = next_item(&d.queue[Q_CHAIN], &slot); it < d.nchain_items;
        it = next_item(&d.queue[Q_CHAIN], &slot)) {
     const int4 item = d.chain_items[it];
-    const int4 rb = d.rblk[item.x];
-    if (d.sg_on && d.sg_state[4 * rb.y + 2] == 1) continue;   // surrogate call: k_sg_grad instead
-    const int t = rb.x;
-    if (t != staged) {
-      stage_tables<ROWS>(d, dyn_tab, t, true);
-      stage_tables<ROWS>(d, dyn_tab + (size_t)d.ntypes * kTblN, t, false);
-      staged = t;
+    const int4 bp = d.cblk[item.x], bq = d.cblk[item.y];
+    if (d.sg_on && d.sg_state[4 * bp.y + 2] == 1) continue;   // surrogate call: k_sg_grad instead
+    const bool diag = item.x == item.y;
+    const int nt = d.ntypes;
+    for (int k = threadIdx.x; k < kTblN; k += 256) {
+      tab_rc[k] = d.tbl[(size_t)(bp.x * nt + bq.x) * kTblN + k];
+      tab_cr[k] = d.tbl[(size_t)(bq.x * nt + bp.x) * kTblN + k];
     }
-    const SysInfo &S = d.sys[rb.y];
-    const int hend = S.a0 + S.nh;                 // heavy atoms of the system end here
-    const int kr = d.row_perm[item.x * ROWS + threadIdx.x];
-    float4 pk = make_float4(0.f, 0.f, 0.f, 1.f);
+    const SysInfo &S = d.sys[bp.y];
+    const int hend = S.a0 + S.nh;
+    {
+      const int j = d.cperm[item.y * 256 + threadIdx.x];
+      float4 p = make_float4(1e6f, 1e6f, 1e6f, 1.f), q = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (j >= 0) {
+        const float4 pj = d.pos[j];
+        const bool hv = j < hend;
+        p = make_float4(pj.x, pj.y, pj.z, 1.0f / pj.w);
+        q = make_float4(hv ? d.sp_f[j] : 0.f, d.q2ab_f[j], d.abrw_f[j], hv ? 1.f : 0.f);
+      }
+      cpos[threadIdx.x] = p;
+      cpar[threadIdx.x] = q;
+    }
+    const int kr = d.cperm[item.x * 256 + threadIdx.x];
+    float4 pk = make_float4(-1e6f, -1e6f, -1e6f, 1.f);
     float spk = 0.f, q2k = 0.f, abk = 0.f;
     bool kheavy = false;
     if (kr >= 0) {
@@ -243,84 +260,84 @@ __global__ __launch_bounds__(ROWS) void k_chain(Dev d) {
       spk = kheavy ? d.sp_f[kr] : 0.f;
     }
     const float iRk = 1.0f / pk.w;
-    const int c0 = item.y, c1 = item.z;
-    double gb[3] = {0, 0, 0}, vw[3] = {0, 0, 0}, U = 0.0, V = 0.0;
-    for (int base = c0; base < c1; base += kTile) {
-      __syncthreads();
-      for (int k = threadIdx.x; k < kTile; k += ROWS) {
-        const int j = base + k;
-        if (j < c1) {
-          const float4 pj = d.pos[j];
-          cpos[k] = make_float4(pj.x, pj.y, pj.z, 1.0f / pj.w);
-          cpar[k] = make_float4(j < hend ? d.sp_f[j] : 0.f, d.q2ab_f[j], d.abrw_f[j], __int_as_float(d.rtype[j]));
-        }
-      }
-      __syncthreads();
-      const int nc = min(kTile, c1 - base);
-      float tg[3] = {0.f, 0.f, 0.f}, tv[3] = {0.f, 0.f, 0.f}, tU = 0.f, tV = 0.f;
-      if (kr >= 0) {
-        for (int c = 0; c < nc; c++) {
-          const int j = base + c;
-          const bool jheavy = j < hend;
-          if (j == kr || (!kheavy && !jheavy)) continue;
-          const float4 pj = cpos[c];
-          const float4 pp = cpar[c];
+    __syncthreads();
+    float rg[3] = {0.f, 0.f, 0.f}, rv[3] = {0.f, 0.f, 0.f}, rU = 0.f, rV = 0.f;
+    for (int c = 0; c < 4; c++) {
+      float ca[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      for (int s = 0; s < 64; s++) {
+        const int col = c * 64 + ((lane - s) & 63);        // the column whose sums this lane holds
+        const float4 pj = cpos[col];
+        const float4 pp = cpar[col];
+        const bool jheavy = pp.w != 0.f;
+        const bool pair = (kr >= 0) && (kheavy || jheavy) && !(diag && col == w * 64 + lane);
+        if (pair) {
           const float ex = pj.x - pk.x, ey = pj.y - pk.y, ez = pj.z - pk.z;
           const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
-          const int tj = __float_as_int(pp.w);
           float hgb = 0.f, hvw = 0.f;
-          if (jheavy) {
-            float q, dq;
-            q4_lookup<false, true>(tab_kj[tj], dist * pj.w, dx_, dxinv, dp1, dp2, sgn2, q, dq);
+          if (jheavy) {                                      // k descreened by j
+            float qv, dq;
+            q4_lookup<true, true>(tab_rc, dist * pj.w, dx_, dxinv, dp1, dp2, sgn2, qv, dq);
+            qv *= pj.w;
             dq *= pj.w * pj.w;
             hgb += q2k * pp.x * dq;
             hvw += abk * pp.x * dq;
+            ca[6] += q2k * qv;
+            ca[7] += abk * qv;
           }
-          if (kheavy) {
-            float q, dq;
-            q4_lookup<true, true>(tab_jk[tj], dist * iRk, dx_, dxinv, dp1, dp2, sgn2, q, dq);
-            q *= iRk;
+          if (kheavy) {                                      // j descreened by k
+            float qv, dq;
+            q4_lookup<true, true>(tab_cr, dist * iRk, dx_, dxinv, dp1, dp2, sgn2, qv, dq);
+            qv *= iRk;
             dq *= iRk * iRk;
             hgb += pp.y * spk * dq;
             hvw += pp.z * spk * dq;
-            tU += pp.y * q;
-            tV += pp.z * q;
+            rU += pp.y * qv;
+            rV += pp.z * qv;
           }
           const float id = 1.0f / dist;
           hgb *= id;
           hvw *= id;
-          tg[0] += hgb * ex; tg[1] += hgb * ey; tg[2] += hgb * ez;
-          tv[0] += hvw * ex; tv[1] += hvw * ey; tv[2] += hvw * ez;
+          rg[0] += hgb * ex; rg[1] += hgb * ey; rg[2] += hgb * ez;
+          rv[0] += hvw * ex; rv[1] += hvw * ey; rv[2] += hvw * ez;
+          ca[0] -= hgb * ex; ca[1] -= hgb * ey; ca[2] -= hgb * ez;
+          ca[3] -= hvw * ex; ca[4] -= hvw * ey; ca[5] -= hvw * ez;
         }
+#pragma unroll
+        for (int q = 0; q < 8; q++) ca[q] = wave_rot(ca[q]);
       }
-      for (int q = 0; q < 3; q++) { gb[q] += tg[q]; vw[q] += tv[q]; }
-      U += tU;
-      V += tV;
+#pragma unroll
+      for (int q = 0; q < 8; q++) colacc[w][q][c * 64 + lane] = ca[q];
     }
     if (kr >= 0) {
-      long long *p = d.acc_ch + 8 * (size_t)kr;
+      long long *pr = d.acc_ch + 8 * (size_t)kr;
       for (int q = 0; q < 3; q++) {
-        fix_add(p + q, gb[q]);
-        fix_add(p + 3 + q, vw[q]);
+        fix_add(pr + q, (double)rg[q]);
+        fix_add(pr + 3 + q, (double)rv[q]);
       }
-      fix_add(p + 6, U);
-      fix_add(p + 7, V);
+      fix_add(pr + 6, (double)rU);
+      fix_add(pr + 7, (double)rV);
+    }
+    __syncthreads();
+    const int j = d.cperm[item.y * 256 + threadIdx.x];
+    if (!diag && j >= 0) {
+      long long *pc = d.acc_ch + 8 * (size_t)j;
+      const int t = threadIdx.x;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        fix_add(pc + q, (double)colacc[0][q][t] + colacc[1][q][t] + colacc[2][q][t] + colacc[3][q][t]);
     }
   }
 }
 
 void launch_chain(const Dev &d, const Tuning &t, hipStream_t s) {
   if (!d.nchain_items) return;
-  const size_t lds = 2 * (size_t)d.ntypes * kTblN * sizeof(float2);
   const int grid = t.chain_grid > 0 ? std::min(t.chain_grid, d.nchain_items) : d.nchain_items;
-  if (d.rows == 64) hipLaunchKernelGGL(k_chain<64>, dim3(grid), dim3(64), lds, s, d);
-  else if (d.rows == 128) hipLaunchKernelGGL(k_chain<128>, dim3(grid), dim3(128), lds, s, d);
-  else hipLaunchKernelGGL(k_chain<256>, dim3(grid), dim3(256), lds, s, d);
+  hipLaunchKernelGGL(k_chain, dim3(grid), dim3(256), 0, s, d);
 }
 
 // allow the spline kernels the dynamic LDS their tables need
 hipError_t prepare_pairs(int ntypes) {
-  const int d1 = ntypes * kTblN * (int)sizeof(float2), d2 = 2 * d1;
+  const int d1 = ntypes * kTblN * (int)sizeof(float2);
   hipError_t e = hipSuccess;
   auto set = [&](const void *f, int bytes) {
     if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -328,9 +345,6 @@ hipError_t prepare_pairs(int ntypes) {
   set((const void *)k_descreen<64>, d1);
   set((const void *)k_descreen<128>, d1);
   set((const void *)k_descreen<256>, d1);
-  set((const void *)k_chain<64>, d2);
-  set((const void *)k_chain<128>, d2);
-  set((const void *)k_chain<256>, d2);
   return e;
 }
 

@@ -54,8 +54,9 @@ int agbnp3_nblist_rebuilds(int tag, int *calls, int *rebuilds);
 int agbnp3_batch_set_nblist_reuse(int batch, float skin, int every);
 
 /* ---- launch shapes and autotuner ----------------------------------------- */
-/* params[7]: rows per spline-kernel block (64, 128, 256); columns per
-   descreening item and per chain-rule item (multiples of 256, 0 = automatic);
+/* params[7]: rows per descreening block (64, 128, 256); columns per
+   descreening item (multiple of 256, 0 = automatic); reserved (0; the chain
+   rule works on 256 x 256 tile pairs);
    persistent grid of the descreening, GB-pair and chain-rule kernels (blocks,
    0 = one per work item); neighbour-list row capacity (64, 128, 256). */
 #define AGBNP3_TUNING_PARAMS 7
