@@ -120,7 +120,7 @@ struct SgModel {
 
 // training batch buffers (feature-major activations and deltas)
 struct SgTrain {
-  const unsigned short *X;   // normalised inputs [N][kSgKPad] bf16
+  const float *X;            // normalised inputs [N][kSgKPad]
   const float *y;            // normalised targets [N]
   unsigned short *A0, *A1, *A2, *D1, *D2, *D3;   // [features][B] bf16
   float *A3, *dy, *err2;
@@ -216,7 +216,7 @@ struct Dev {
   const int2 *sg_rows;                 // 128-atom row blocks inside systems
   int nsg_rows;
   float *sg_F;                         // raw shell descriptors [n][64]
-  unsigned short *sg_X;                // normalised bf16 input [n][kSgKPad]
+  float *sg_X;                         // normalised input [n][kSgKPad]
   float *sg_s, *sg_dsdF;               // predicted s [n], ds/dF [n][64]
   float *sg_target;                    // plain descreening sum of the call [n]
   double *sg_c, *sg_delta;             // anchor corrections, surrogate error at the anchor [n]
@@ -254,8 +254,7 @@ void launch_sg_grad(const Dev &d, hipStream_t s);
 void launch_sg_judge(const Dev &d, hipStream_t s);
 hipError_t prepare_surrogate();
 void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t s);
-void launch_sg_infer(const SgModel &m, const unsigned short *X, int n, float *s_out, float *dsdF, int grid,
-                     hipStream_t s);
+void launch_sg_infer(const SgModel &m, const float *X, int n, float *s_out, float *dsdF, int grid, hipStream_t s);
 size_t sg_mlp_lds();
 void launch_sg_train_step(const SgModel &m, const SgTrain &t, int r0, int B, float *grad, float *loss,
                           int grid, hipStream_t s);

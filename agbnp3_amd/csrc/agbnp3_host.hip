@@ -207,7 +207,7 @@ struct SgHost {
   // training set
   int cap = 0, count = 0;
   float *F = nullptr, *target = nullptr;          // raw [cap][66], [cap]
-  unsigned short *X = nullptr;                    // normalised bf16 [cap][96]
+  float *X = nullptr;                             // normalised [cap][96]
   float *y = nullptr;                             // normalised targets [cap]
   int bcap = 0;
   SgTrain tb{};
@@ -470,10 +470,15 @@ int build_items(Pack &P) {
   std::vector<int4> cblk;
   std::vector<int> csize;
   std::vector<std::pair<int, int>> csys;             // first and end block of each system
+  // types in order of radius: the blocks (and so which side of a tile pair an
+  // atom is on) do not depend on how a pack numbers its radius types
+  std::vector<int> by_radius(P.ntypes);
+  for (int t = 0; t < P.ntypes; t++) by_radius[t] = t;
+  std::stable_sort(by_radius.begin(), by_radius.end(), [&](int a, int b) { return P.radii[a] < P.radii[b]; });
   for (int b = 0; b < P.nsys; b++) {
     const SysInfo &S = P.sinfo[b];
     const int first = (int)cblk.size();
-    for (int t = 0; t < P.ntypes; t++) {
+    for (int t : by_radius) {
       std::vector<int> r;
       for (int i = S.a0; i < S.a0 + S.n; i++)
         if (P.rtype[i] == t) r.push_back(i);
@@ -1922,19 +1927,19 @@ int agbnp3_sg_fit(int model, float margin, unsigned long long seed) {
   for (int i = 0; i < N; i++) perm[i] = i;
   std::mt19937_64 rng(seed);
   std::shuffle(perm.begin(), perm.end(), rng);
-  std::vector<unsigned short> X((size_t)N * kSgKPad, 0);
+  std::vector<float> X((size_t)N * kSgKPad, 0.f);
   std::vector<float> y(N);
   for (int r = 0; r < N; r++) {
     const int i = perm[r];
     for (int k = 0; k < kSgNFeat; k++)
-      X[(size_t)r * kSgKPad + k] = host_bf16((F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k]);
+      X[(size_t)r * kSgKPad + k] = (F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k];
     y[r] = (float)((t[i] - ym) / M->ysd);
   }
   HIPCHK(hipStreamSynchronize(M->stream));
   for (void *q : M->tallocs) (void)hipFree(q);
   M->tallocs.clear();
   M->bcap = 0;
-  if (upload(M->tallocs, (const unsigned short **)&M->X, X) || upload(M->tallocs, (const float **)&M->y, y))
+  if (upload(M->tallocs, (const float **)&M->X, X) || upload(M->tallocs, (const float **)&M->y, y))
     return AGBNP_ERR;
   HIPCHK(hipMemcpy(M->norm, norm.data(), sizeof(float) * norm.size(), hipMemcpyHostToDevice));
   sg_refresh_attached(model);
@@ -2014,12 +2019,12 @@ int agbnp3_sg_infer(int model, const float *F, int n, float *s_out, float *dsdF)
   std::vector<float> norm(4 * kSgNFeat);
   HIPCHK(hipStreamSynchronize(M->stream));
   HIPCHK(hipMemcpy(norm.data(), M->norm, sizeof(float) * norm.size(), hipMemcpyDeviceToHost));
-  std::vector<unsigned short> X((size_t)n * kSgKPad, 0);
+  std::vector<float> X((size_t)n * kSgKPad, 0.f);
   for (int i = 0; i < n; i++)
     for (int k = 0; k < kSgNFeat; k++)
-      X[(size_t)i * kSgKPad + k] = host_bf16((F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k]);
+      X[(size_t)i * kSgKPad + k] = (F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k];
   std::vector<void *> tmp;
-  const unsigned short *dX = nullptr;
+  const float *dX = nullptr;
   float *ds = nullptr, *dd = nullptr;
   int rc = AGBNP_ERR;
   if (!upload(tmp, &dX, X) && !dalloc(tmp, &ds, (size_t)n) && !dalloc(tmp, &dd, (size_t)n * 64)) {

@@ -8,14 +8,18 @@
 // Descriptors (per atom i): four heavy-atom channels (by R) x 16 radial
 // shells up to 12 A, F[c][k] = sum_j B3((d_ij - mu_k)/h) T(d_ij) with
 // h = 0.75 A, mu_k = h (k + 1/2), B3 the cubic B-spline and T a cosine taper
-// on [10.5, 12] A; then R_i and q_i.  Normalised and rounded to bf16 they are
-// the MLP input, padded to 96 columns.
+// on [10.5, 12] A; then R_i and q_i.  Normalised (FP32) they are the MLP
+// input, padded to 96 columns.
 //
-// MLP: 66 -> 128 -> 128 -> 128 -> 1, SiLU, bf16 operands on the matrix cores
-// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation and biases.  One block
-// (4 waves) holds the three weight matrices in LDS and runs 32-atom tiles:
-// forward, then the input gradient ds/dF back through the layers (the
-// transposed products read the row-major weights from L2).
+// MLP: 66 -> 128 -> 128 -> 128 -> 1, SiLU, on the matrix cores
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation and biases.  Weights are
+// bf16; inputs and hidden activations are carried as a bf16 pair (hi + lo,
+// about 16 significant bits, two products per layer), so the model is a
+// smooth function of the coordinates rather than one that steps with bf16
+// rounding.  One block (4 waves) holds the three weight matrices in LDS and
+// runs 32-atom tiles: forward, then the input gradient ds/dF back through
+// the layers with bf16 deltas (the transposed products read the row-major
+// weights from L2).
 //
 // MFMA 16x16x32 lane layout: lane l holds A[l%16][8(l/16)+r],
 // B[8(l/16)+r][l%16] (r = 0..7) and D[4(l/16)+r][l%16] (r = 0..3).
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(kSgRows) void k_sg_features(Dev d) {
   if (!row) return;
   // raw shell sums, normalised bf16 input row, in-range test
   bool oor = false;
-  unsigned short *x = d.sg_X + (size_t)i * kSgKPad;
+  float *x = d.sg_X + (size_t)i * kSgKPad;
   for (int k = 0; k < kSgKPad; k++) {
     float v = 0.f;
     if (k < kSgNFeat) {
@@ -123,15 +127,16 @@ __global__ __launch_bounds__(kSgRows) void k_sg_features(Dev d) {
       v = (raw - d.sg.xmu[k]) * d.sg.xsd_inv[k];
       oor |= !(v >= d.sg.xlo[k] && v <= d.sg.xhi[k]);
     }
-    x[k] = __builtin_bit_cast(unsigned short, (__bf16)v);
+    x[k] = v;
   }
   if (oor) d.sg_state[4 * sys + 3] = 1;
 }
 
 // ---------------------------------------------------------------- the MLP
-// Z[32][128] = A[32][K] W + b with W staged transposed ([out][K]) in LDS
+// Z[32][128] = (Ahi + Alo)[32][K] W + b with W staged transposed ([out][K]) in LDS
 template <int K, int LDA, int LDW>
-__device__ __forceinline__ void mlp_layer(const __bf16 *A, const __bf16 *Wt, const float *bias, float *Z) {
+__device__ __forceinline__ void mlp_layer(const __bf16 *Ahi, const __bf16 *Alo, const __bf16 *Wt, const float *bias,
+                                          float *Z) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int t = 0; t < 4; t++) {
@@ -139,9 +144,12 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *A, const __bf16 *Wt, con
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k0 = 0; k0 < K; k0 += 32) {
-      const bf16x8 a = *(const bf16x8 *)(A + (mt * 16 + (lane & 15)) * LDA + k0 + 8 * (lane >> 4));
+      const int ao = (mt * 16 + (lane & 15)) * LDA + k0 + 8 * (lane >> 4);
+      const bf16x8 ah = *(const bf16x8 *)(Ahi + ao);
+      const bf16x8 al = *(const bf16x8 *)(Alo + ao);
       const bf16x8 b = *(const bf16x8 *)(Wt + (nt * 16 + (lane & 15)) * LDW + k0 + 8 * (lane >> 4));
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b, acc, 0, 0, 0);
     }
     const int col = nt * 16 + (lane & 15);
     const float bc = bias ? bias[col] : 0.f;
@@ -171,11 +179,20 @@ __device__ __forceinline__ void mlp_back(const __bf16 *D, const unsigned short *
 
 struct MlpLds {
   __bf16 wt1[kSgHid * kLd1], wt2[kSgHid * kLdH], wt3[kSgHid * kLdH];
-  __bf16 act[kSgTile * kLdH];
+  __bf16 act[kSgTile * kLdH];          // activation hi part; the deltas on the way back
+  __bf16 alo[kSgTile * kLdH];          // activation lo part; then output partials and seeds
   float z1[kSgTile * kSgHid], z2[kSgTile * kSgHid], z3[kSgTile * kSgHid];
-  float part[kSgTile][8];
-  float seed[kSgTile];
+  __device__ float *part() { return reinterpret_cast<float *>(alo); }              // [kSgTile][8]
+  __device__ float *seed() { return reinterpret_cast<float *>(alo) + kSgTile * 8; }  // [kSgTile]
 };
+
+static_assert(sizeof(MlpLds) <= 160 * 1024, "MLP tile state exceeds the LDS of one workgroup");
+
+// v as the bf16 pair (hi, lo), hi + lo = v to about 16 significant bits
+__device__ __forceinline__ void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
 
 __device__ void stage_weights(const SgModel &m, MlpLds &L) {
   for (int k = threadIdx.x; k < kSgHid * kSgKPad; k += 256) {
@@ -189,46 +206,53 @@ __device__ void stage_weights(const SgModel &m, MlpLds &L) {
   }
 }
 
-// Forward for rows [r0, r0 + 32) of X (bf16, kSgKPad columns), z1..z3 left
+// Forward for rows [r0, r0 + 32) of X (FP32, kSgKPad columns), z1..z3 left
 // in LDS; returns the normalised output y for thread t < 32 (fixed-order sum)
-__device__ float mlp_forward_tile(const SgModel &m, MlpLds &L, const unsigned short *X, int r0, int nrows) {
+__device__ float mlp_forward_tile(const SgModel &m, MlpLds &L, const float *X, int r0, int nrows) {
   for (int k = threadIdx.x; k < kSgTile * kSgKPad; k += 256) {
     const int r = k / kSgKPad, c = k - r * kSgKPad;
-    L.act[r * kLd1 + c] = r < nrows ? __builtin_bit_cast(__bf16, X[(size_t)(r0 + r) * kSgKPad + c]) : (__bf16)0.f;
+    const float v = r < nrows ? X[(size_t)(r0 + r) * kSgKPad + c] : 0.f;
+    split_bf16(v, L.act[r * kLd1 + c], L.alo[r * kLd1 + c]);
   }
   __syncthreads();
-  mlp_layer<kSgKPad, kLd1, kLd1>(L.act, L.wt1, m.b1, L.z1);
+  mlp_layer<kSgKPad, kLd1, kLd1>(L.act, L.alo, L.wt1, m.b1, L.z1);
   __syncthreads();
-  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
-    L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)silu(L.z1[k]);
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
+    const int o = (k / kSgHid) * kLdH + (k % kSgHid);
+    split_bf16(silu(L.z1[k]), L.act[o], L.alo[o]);
+  }
   __syncthreads();
-  mlp_layer<kSgHid, kLdH, kLdH>(L.act, L.wt2, m.b2, L.z2);
+  mlp_layer<kSgHid, kLdH, kLdH>(L.act, L.alo, L.wt2, m.b2, L.z2);
   __syncthreads();
-  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
-    L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)silu(L.z2[k]);
+  for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
+    const int o = (k / kSgHid) * kLdH + (k % kSgHid);
+    split_bf16(silu(L.z2[k]), L.act[o], L.alo[o]);
+  }
   __syncthreads();
-  mlp_layer<kSgHid, kLdH, kLdH>(L.act, L.wt3, m.b3, L.z3);
+  mlp_layer<kSgHid, kLdH, kLdH>(L.act, L.alo, L.wt3, m.b3, L.z3);
   __syncthreads();
+  float *part = L.part();
   {
     const int r = threadIdx.x >> 3, p = threadIdx.x & 7;
     float ps = 0.f;
     for (int c = 16 * p; c < 16 * p + 16; c++) ps += silu(L.z3[r * kSgHid + c]) * m.w4[c];
-    L.part[r][p] = ps;
+    part[r * 8 + p] = ps;
   }
   __syncthreads();
   float y = 0.f;
   if (threadIdx.x < kSgTile) {
     y = *m.b4;
-    for (int p = 0; p < 8; p++) y += L.part[threadIdx.x][p];
+    for (int p = 0; p < 8; p++) y += part[threadIdx.x * 8 + p];
   }
   return y;
 }
 
-// delta3 = seed_r w4 silu'(z3) into act (seed[] set by the caller)
+// delta3 = seed_r w4 silu'(z3) into act (seed() set by the caller)
 __device__ void mlp_seed_delta3(const SgModel &m, MlpLds &L) {
+  const float *seed = L.seed();
   for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
     const int r = k / kSgHid, c = k % kSgHid;
-    L.act[r * kLdH + c] = (__bf16)(L.seed[r] * m.w4[c] * silup(L.z3[k]));
+    L.act[r * kLdH + c] = (__bf16)(seed[r] * m.w4[c] * silup(L.z3[k]));
   }
   __syncthreads();
 }
@@ -270,7 +294,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
     const float y = mlp_forward_tile(m, L, d.sg_X, r0, nrows);
     if (threadIdx.x < kSgTile) {
       if (threadIdx.x < nrows) d.sg_s[r0 + threadIdx.x] = y * m.ysd + m.ymu;
-      L.seed[threadIdx.x] = 1.0f;
+      L.seed()[threadIdx.x] = 1.0f;
     }
     __syncthreads();
     mlp_seed_delta3(m, L);
@@ -303,7 +327,7 @@ __global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r
         t.dy[b0 + threadIdx.x] = dy;
         t.err2[b0 + threadIdx.x] = e * e;
       }
-      L.seed[threadIdx.x] = dy;
+      L.seed()[threadIdx.x] = dy;
     }
     __syncthreads();
     // activations (feature-major): X, a1, a2 as bf16; a3 in FP32
@@ -317,7 +341,8 @@ __global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r
     }
     for (int k = threadIdx.x; k < kSgTile * kSgKPad; k += 256) {
       const int r = k / kSgKPad, c = k % kSgKPad;
-      if (r < nrows) t.A0[(size_t)c * B + b0 + r] = t.X[(size_t)(r0 + b0 + r) * kSgKPad + c];
+      if (r < nrows)
+        t.A0[(size_t)c * B + b0 + r] = __builtin_bit_cast(unsigned short, (__bf16)t.X[(size_t)(r0 + b0 + r) * kSgKPad + c]);
     }
     mlp_seed_delta3(m, L);
     mlp_backward_tile(m, L, false, [&](int layer) {
@@ -611,9 +636,8 @@ void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t
   hipLaunchKernelGGL(k_sg_append, dim3((d.n + 255) / 256), dim3(256), 0, s, d, F, target, at);
 }
 
-// inference on given inputs (tests): X [n][kSgKPad] normalised bf16
-__global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const unsigned short *X, int n, float *s_out,
-                                                  float *dsdF) {
+// inference on given inputs (tests): X [n][kSgKPad] normalised FP32
+__global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const float *X, int n, float *s_out, float *dsdF) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
   stage_weights(m, L);
@@ -624,7 +648,7 @@ __global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const unsigned shor
     const float y = mlp_forward_tile(m, L, X, r0, nrows);
     if (threadIdx.x < kSgTile) {
       if (threadIdx.x < nrows) s_out[r0 + threadIdx.x] = y * m.ysd + m.ymu;
-      L.seed[threadIdx.x] = 1.0f;
+      L.seed()[threadIdx.x] = 1.0f;
     }
     __syncthreads();
     mlp_seed_delta3(m, L);
@@ -636,8 +660,7 @@ __global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const unsigned shor
   }
 }
 
-void launch_sg_infer(const SgModel &m, const unsigned short *X, int n, float *s_out, float *dsdF, int grid,
-                     hipStream_t s) {
+void launch_sg_infer(const SgModel &m, const float *X, int n, float *s_out, float *dsdF, int grid, hipStream_t s) {
   const int ntiles = (n + kSgTile - 1) / kSgTile;
   hipLaunchKernelGGL(k_sg_infer, dim3(std::min(ntiles, grid)), dim3(256), sizeof(MlpLds), s, m, X, n, s_out, dsdF);
 }

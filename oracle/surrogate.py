@@ -11,10 +11,11 @@ checker for the GPU kernels of agbnp3_amd/csrc/k_surrogate.hip:
   |t| < 2) and T a cosine taper from 1 at 10.5 A to 0 at 12 A; then R_i and
   q_i (66 features);
 - the MLP 66 -> 128 -> 128 -> 128 -> 1 with SiLU, evaluated as the GPU does:
-  bf16 operands (normalised inputs, weights, hidden activations), products
-  accumulated exactly (here in FP64; FP32 on the GPU), FP32 biases;
-- its input gradient and, for training, the weight gradients of a mean
-  squared error.
+  bf16 weights; normalised inputs and hidden activations carried as a pair
+  of bf16 values (hi = bf16(v), lo = bf16(v - hi)); products accumulated
+  exactly (here in FP64; FP32 on the GPU), FP32 biases;
+- its input gradient (bf16 deltas) and, for training, the weight gradients
+  of a mean squared error (bf16 activations and deltas in the products).
 
 Everything is plain numpy in FP64 on bf16-rounded values.
 """
@@ -152,20 +153,28 @@ def exact(a):
     return np.asarray(a, np.float64)
 
 
-def normalise(P, F, rnd=bf16):
+def split(a):
+    """The value the bf16 pair (hi, lo) carries: hi = bf16(v), lo = bf16(v - hi)."""
+    a = np.asarray(a, np.float32).astype(np.float64)
+    hi = bf16(a)
+    return hi + bf16(a - hi)
+
+
+def normalise(P, F, act=split):
     X = np.zeros((len(F), KPAD))
     X[:, :NFEAT] = (F - P.xmu) / P.xsd
-    return rnd(X)
+    return act(X)
 
 
-def forward(P, F, rnd=bf16):
-    """-> (s, cache) with s the descreening sum predicted for each atom."""
-    X = normalise(P, F, rnd)
+def forward(P, F, rnd=bf16, act=split):
+    """-> (s, cache) with s the descreening sum predicted for each atom;
+    rnd rounds the weights, act the inputs and hidden activations."""
+    X = normalise(P, F, act)
     W1 = rnd(np.vstack([P.W1, np.zeros((KPAD - NFEAT, HID))]))
     z1 = X @ W1 + P.b1
-    a1 = rnd(silu(z1))
+    a1 = act(silu(z1))
     z2 = a1 @ rnd(P.W2) + P.b2
-    a2 = rnd(silu(z2))
+    a2 = act(silu(z2))
     z3 = a2 @ rnd(P.W3) + P.b3
     a3 = silu(z3)
     y = a3 @ P.w4 + P.b4
@@ -182,9 +191,11 @@ def input_gradient(P, cache, rnd=bf16):
     return dX * P.ysd / P.xsd
 
 
-def loss_and_weight_gradients(P, F, target, rnd=bf16):
-    """Mean squared error of the normalised output and its parameter gradients."""
-    s, (X, z1, a1, z2, a2, z3, a3, y) = forward(P, F, rnd)
+def loss_and_weight_gradients(P, F, target, rnd=bf16, act=split):
+    """Mean squared error of the normalised output and its parameter gradients;
+    the weight-gradient products take the activations rounded by rnd."""
+    s, (X, z1, a1, z2, a2, z3, a3, y) = forward(P, F, rnd, act)
+    X, a1, a2 = rnd(X), rnd(a1), rnd(a2)
     t = (target - P.ymu) / P.ysd
     n = len(t)
     e = y - t

@@ -86,14 +86,14 @@ def random_params(seed):
 def test_mlp_input_gradient_matches_finite_differences():
     P = random_params(4)
     F = np.random.default_rng(5).normal(size=(6, 66))
-    s, cache = sg.forward(P, F, sg.exact)
+    s, cache = sg.forward(P, F, sg.exact, sg.exact)
     dF = sg.input_gradient(P, cache, sg.exact)
     h = 1e-6
     for k in (0, 13, 40, 64, 65):
         F1, F2 = F.copy(), F.copy()
         F1[:, k] += h
         F2[:, k] -= h
-        fd = (sg.forward(P, F1, sg.exact)[0] - sg.forward(P, F2, sg.exact)[0]) / (2 * h)
+        fd = (sg.forward(P, F1, sg.exact, sg.exact)[0] - sg.forward(P, F2, sg.exact, sg.exact)[0]) / (2 * h)
         assert np.allclose(fd, dF[:, k], rtol=1e-5, atol=1e-8)
 
 
@@ -102,15 +102,24 @@ def test_mlp_weight_gradients_match_finite_differences():
     rng = np.random.default_rng(7)
     F = rng.normal(size=(9, 66))
     t = rng.normal(5.0, 1.0, 9)
-    loss, g = sg.loss_and_weight_gradients(P, F, t, sg.exact)
+    loss, g = sg.loss_and_weight_gradients(P, F, t, sg.exact, sg.exact)
     h = 1e-6
     for name, idx in (("W1", (3, 5)), ("b1", (7,)), ("W2", (10, 100)), ("W3", (127, 0)), ("b3", (64,)),
                       ("w4", (12,))):
         arr = getattr(P, name)
         old = arr[idx]
         arr[idx] = old + h
-        l1 = sg.loss_and_weight_gradients(P, F, t, sg.exact)[0]
+        l1 = sg.loss_and_weight_gradients(P, F, t, sg.exact, sg.exact)[0]
         arr[idx] = old - h
-        l2 = sg.loss_and_weight_gradients(P, F, t, sg.exact)[0]
+        l2 = sg.loss_and_weight_gradients(P, F, t, sg.exact, sg.exact)[0]
         arr[idx] = old
         assert abs((l1 - l2) / (2 * h) - g[name][idx]) < 1e-6 * max(1.0, abs(g[name][idx])), name
+
+
+def test_bf16_pair_carries_sixteen_bits():
+    rng = np.random.default_rng(12)
+    v = rng.normal(0, 3, 10000).astype(np.float32).astype(np.float64)
+    hi = sg.bf16(v)
+    assert np.all(np.abs(hi - v) <= 2.0 ** -8 * np.abs(v))          # one bf16: 8 significant bits
+    assert np.all(np.abs(sg.split(v) - v) <= 2.0 ** -16 * np.abs(v))   # the pair: 16
+    assert np.array_equal(sg.split(hi), hi)                           # bf16 values are carried exactly
