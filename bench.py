@@ -65,6 +65,24 @@ def algorithmic_flops(n, nh):
     }
 
 
+PROFILE_JSON = os.path.join(HERE, "profiles", "c1_kernels_25k.json")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a copy of
+    known size), or None."""
+    try:
+        with open(PROFILE_JSON) as f:
+            kernels = json.load(f)["kernels"]
+    except (OSError, KeyError, ValueError):
+        return None
+    for name, k in kernels.items():
+        if name == "k_" + kernel or name.startswith("k_" + kernel + "<"):
+            return k["hbm_bytes"]
+    return None
+
+
 def cpu_baseline(system, calls, threads):
     import oracle
     work = tempfile.mkdtemp(prefix="agb_cpu_")
@@ -457,8 +475,10 @@ def main():
     timed = {k: ktimes[k] for k in flops if k in ktimes}
     dom = max(timed, key=timed.get)
     achieved = flops[dom] / (timed[dom] * 1e-3)
+    traffic = profiled_traffic(dom) if n == 25000 and args.seed == 20260025 else None
     roofline = dict(kernel=dom, bound="valu-fp32", achieved=achieved / 1e12, peak=PEAK_FP32 / 1e12,
-                    unit="TFLOP/s", frac=achieved / PEAK_FP32, traffic=None,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP32, traffic=traffic,
+                    traffic_unit="bytes per launch", traffic_source=os.path.relpath(PROFILE_JSON, HERE),
                     avg_ms=timed[dom], flops_per_launch=flops[dom])
     total_flops = sum(flops.values())
 
