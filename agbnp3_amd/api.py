@@ -36,8 +36,9 @@ EXPORTED = (
     "agbnp3_set_nblist_reuse", "agbnp3_batch_set_nblist_reuse", "agbnp3_nblist_rebuilds",
     "agbnp3_sg_new", "agbnp3_sg_delete", "agbnp3_sg_size", "agbnp3_sg_get_params", "agbnp3_sg_set_params",
     "agbnp3_sg_reserve", "agbnp3_batch_sg_collect", "agbnp3_sg_dataset", "agbnp3_sg_fit", "agbnp3_sg_train",
+    "agbnp3_sg_set_loss", "agbnp3_sg_train_rows",
     "agbnp3_sg_grads", "agbnp3_sg_infer", "agbnp3_sg_attach", "agbnp3_batch_sg_attach", "agbnp3_sg_stats",
-    "agbnp3_batch_sg_stats",
+    "agbnp3_batch_sg_stats", "agbnp3_sg_lists", "agbnp3_batch_sg_lists",
 )
 TUNING_PARAMS = ("rows", "desc_chunk", "chain_chunk", "desc_grid", "gb_grid", "chain_grid", "cap")
 NBLIST_EXPORTED = ("nblist_reset_neighbor_list", "nblist_delete_neighbor_list", "nblist_reallocate_neighbor_list")
@@ -129,12 +130,16 @@ def load():
         L.agbnp3_sg_dataset.argtypes = [ci, ctypes.POINTER(ci), vp, vp]
         L.agbnp3_sg_fit.argtypes = [ci, f32, u64]
         L.agbnp3_sg_train.argtypes = [ci, ci, ci, f32, f32, u64, vp]
+        L.agbnp3_sg_set_loss.argtypes = [ci, f32, f32]
+        L.agbnp3_sg_train_rows.argtypes = [ci, vp, vp, vp]
         L.agbnp3_sg_grads.argtypes = [ci, ci, ci, vp, vp]
         L.agbnp3_sg_infer.argtypes = [ci, vp, ci, vp, vp]
         L.agbnp3_sg_attach.argtypes = [ci, ci, ci]
         L.agbnp3_batch_sg_attach.argtypes = [ci, ci, ci]
         L.agbnp3_sg_stats.argtypes = [ci, vp, vp]
         L.agbnp3_batch_sg_stats.argtypes = [ci, vp, vp]
+        L.agbnp3_sg_lists.argtypes = [ci, vp, vp]
+        L.agbnp3_batch_sg_lists.argtypes = [ci, vp, vp]
         _nb.nblist_reallocate_neighbor_list.argtypes = [ctypes.POINTER(NeighList), ci, ci]
         _nb.nblist_reset_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
         _nb.nblist_delete_neighbor_list.argtypes = [ctypes.POINTER(NeighList)]
@@ -346,6 +351,13 @@ class Instance:
             raise RuntimeError("agbnp3_sg_stats failed")
         return float(err[0]), int(mode[0])
 
+    def sg_lists(self):
+        """(descriptor neighbour-list builds, overflow flag) since sg_attach."""
+        b, o = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        if load().agbnp3_sg_lists(self.tag, _p(b), _p(o)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_lists failed")
+        return int(b[0]), int(o[0])
+
     def nblist_rebuilds(self):
         """(calls, candidate rebuilds) since set_nblist_reuse."""
         c, r = ctypes.c_int(), ctypes.c_int()
@@ -438,6 +450,13 @@ class Batch:
             raise RuntimeError("agbnp3_batch_sg_stats failed")
         return err, mode
 
+    def sg_lists(self):
+        """Per system: (descriptor neighbour-list builds, overflow flags) since sg_attach."""
+        b, o = np.zeros(self.nsys, np.int32), np.zeros(self.nsys, np.int32)
+        if load().agbnp3_batch_sg_lists(self.id, _p(b), _p(o)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_batch_sg_lists failed")
+        return b, o
+
     def set_nblist_reuse(self, skin, every=0):
         if load().agbnp3_batch_set_nblist_reuse(self.id, float(skin), int(every)) != AGBNP_OK:
             raise RuntimeError("agbnp3_batch_set_nblist_reuse failed")
@@ -508,6 +527,21 @@ class Surrogate:
     def fit(self, margin=0.05, seed=0):
         if load().agbnp3_sg_fit(self.id, float(margin), int(seed)) != AGBNP_OK:
             raise RuntimeError("agbnp3_sg_fit failed")
+
+    def set_loss(self, w_abs=1.0, w_diff=1.0):
+        if load().agbnp3_sg_set_loss(self.id, float(w_abs), float(w_diff)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_set_loss failed")
+
+    def train_rows(self):
+        """(dataset row of each training row, pair-difference scale c)."""
+        n = np.zeros(1, np.int32)
+        if load().agbnp3_sg_train_rows(self.id, _p(n), None, None) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_train_rows failed")
+        rows = np.zeros(int(n[0]), np.int32)
+        c = np.zeros(1, np.float32)
+        if load().agbnp3_sg_train_rows(self.id, _p(n), _p(rows), _p(c)) != AGBNP_OK:
+            raise RuntimeError("agbnp3_sg_train_rows failed")
+        return rows, float(c[0])
 
     def train(self, epochs, batch=4096, lr=2e-3, decay=0.9, seed=0):
         out = np.zeros(epochs, np.float32)

@@ -97,6 +97,12 @@ struct SysInfo {
 
 // ---- Born-radius surrogate (MLP 66 -> 128 -> 128 -> 128 -> 1)
 constexpr int kSgNShell = 16, kSgNFeat = 66, kSgKPad = 96, kSgHid = 128;
+constexpr int kSgBlk = 64;               // atoms per descriptor row block
+constexpr int kSgNbrCap = 1536;          // neighbours per atom within kSgCut + kSgSkin
+constexpr float kSgCut = 12.0f, kSgSkin = 1.5f;
+// largest grid of the descreening and chain-rule kernels while a surrogate
+// is attached (their items are then often all skipped)
+constexpr int kSgPersistentGrid = 2048;
 // FP32 master parameter layout: W1 [66][128], b1, W2 [128][128], b2, W3, b3, w4 [128], b4
 constexpr int kSgOffW1 = 0;
 constexpr int kSgOffB1 = kSgOffW1 + kSgNFeat * kSgHid;
@@ -124,6 +130,8 @@ struct SgTrain {
   const float *y;            // normalised targets [N]
   unsigned short *A0, *A1, *A2, *D1, *D2, *D3;   // [features][B] bf16
   float *A3, *dy, *err2;
+  // loss per row w_abs e^2 + w_diff (e - e_partner)^2, rows 2k and 2k+1 a pair
+  float w_abs, w_diff;
 };
 
 // work-queue heads (cleared by the ingest kernel of every call)
@@ -213,8 +221,21 @@ struct Dev {
   SgModel sg;
   const int *sys_sg;                   // per system: model attached
   int *sg_state;
-  const int2 *sg_rows;                 // 128-atom row blocks inside systems
+  int *sg_any_plain;                   // [0]: some system of the pack runs the plain kernels this call
+  const int2 *sg_rows;                 // kSgBlk-slot row blocks inside systems (slots of sg_catoms)
   int nsg_rows;
+  // descriptor neighbour lists: every pair within kSgCut + kSgSkin, kept
+  // while no atom of the system has moved more than kSgSkin / 2 since.  Rows
+  // are the atoms in cell order (sg_catoms, from the same build); row r of
+  // block b: neighbour k at sg_nbr[(b * kSgBlk + r) * kSgNbrCap + k],
+  // heavy neighbours from k = 0, hydrogens (heavy rows only) down from
+  // k = kSgNbrCap - 1; sg_nn[a] = (heavy, hydrogen) counts.
+  int *sg_nbr;
+  int2 *sg_nn;
+  float4 *sg_ref;                      // [n] positions the lists were built from
+  int *sg_nbl;                         // per system [0] built, [1] rebuild this call, [2] builds, [3] overflow
+  GridParams *sg_grid_p;               // per system all-atom cell grid
+  int *sg_cstart, *sg_catoms, *sg_ctmp;  // [nsys][kMaxCells + 1], [n], [n]
   float *sg_F;                         // raw shell descriptors [n][64]
   float *sg_X;                         // normalised input [n][kSgKPad]
   float *sg_s, *sg_dsdF;               // predicted s [n], ds/dF [n][64]

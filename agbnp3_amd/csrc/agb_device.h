@@ -63,6 +63,135 @@ __device__ __forceinline__ int cell_index(const GridParams &g, float x, float y,
   return (cz * g.dim[1] + cy) * g.dim[0] + cx;
 }
 
+// Bounding box of atoms [a0, a0 + count) by one 1024-thread block: lo[3],
+// hi[3] valid in thread 0 (box is block-shared scratch [6][16]); empty
+// ranges give lo = +inf, hi = -inf.
+__device__ __forceinline__ void block_bbox(const float4 *pos, int a0, int count, float (*box)[16], float *lo,
+                                           float *hi) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int k = t; k < count; k += 1024) {
+    const float4 p = pos[a0 + k];
+    l[0] = fminf(l[0], p.x); l[1] = fminf(l[1], p.y); l[2] = fminf(l[2], p.z);
+    h[0] = fmaxf(h[0], p.x); h[1] = fmaxf(h[1], p.y); h[2] = fmaxf(h[2], p.z);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float a = wave_min(l[k]), b = wave_max(h[k]);
+    if (lane == 0) { box[k][w] = a; box[3 + k][w] = b; }
+  }
+  __syncthreads();
+  if (t == 0)
+    for (int k = 0; k < 3; k++) {
+      lo[k] = box[k][0];
+      hi[k] = box[3 + k][0];
+      for (int q = 1; q < 16; q++) { lo[k] = fminf(lo[k], box[k][q]); hi[k] = fmaxf(hi[k], box[3 + k][q]); }
+    }
+}
+
+// Grid over the box [lo, hi] with cells of at least `cs`, grown by 5 % steps
+// until it has at most kMaxCells cells.
+__device__ __forceinline__ GridParams make_grid(const float *lo_in, const float *hi_in, float cs) {
+  GridParams g;
+  float ext[3];
+  for (int k = 0; k < 3; k++) {
+    float lo = lo_in[k], hi = hi_in[k];
+    if (!(hi >= lo)) { lo = 0.f; hi = 0.f; }    // no atoms
+    g.lo[k] = lo - 1e-3f;
+    ext[k] = hi - lo + 2e-3f;
+    if (!(ext[k] < 1e7f)) ext[k] = 1e7f;          // non-finite input: keep the loop finite
+  }
+  for (;;) {
+    long long nc = 1;
+    for (int k = 0; k < 3; k++) {
+      g.dim[k] = (int)floorf(ext[k] / cs) + 1;
+      nc *= g.dim[k];
+    }
+    if (nc <= kMaxCells) { g.ncell = (int)nc; break; }
+    cs *= 1.05f;
+  }
+  g.cs = cs;
+  g.inv_cs = 1.0f / cs;
+  return g;
+}
+
+// Counting sort of atoms [a0, a0 + count) into the cells of g by one
+// 1024-thread block: start[c] is the first slot of cell c (start[ncell] =
+// a0 + count) and out[start[c] ..] its atoms.  cnt (kMaxCells ints), wsum
+// and wpre (16 ints) are block-shared scratch, cell_of[a0 ..] global scratch.
+// The order inside a cell follows the arrival of the atomic adds; with
+// `sorted` each cell is then put in atom order.
+__device__ __forceinline__ void bin_atoms(const GridParams &g, const float4 *pos, int a0, int count, int *cell_of,
+                                          int *start, int *out, int *cnt, int *wsum, int *wpre, bool sorted) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int ncell = g.ncell;
+  for (int c = t; c < ncell; c += 1024) cnt[c] = 0;
+  __syncthreads();
+  for (int k = t; k < count; k += 1024) {
+    const float4 p = pos[a0 + k];
+    const int c = cell_index(g, p.x, p.y, p.z);
+    cell_of[a0 + k] = c;
+    atomicAdd(&cnt[c], 1);
+  }
+  __syncthreads();
+  // exclusive scan of cnt over the cells, contiguous cells per thread
+  const int per = (ncell + 1023) / 1024;
+  int local = 0;
+  for (int k = 0; k < per; k++) {
+    const int c = t * per + k;
+    if (c < ncell) local += cnt[c];
+  }
+  const int incl = wave_incl_scan(local);
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (t < 16) {
+    int s = 0;
+    for (int q = 0; q < t; q++) s += wsum[q];
+    wpre[t] = s;
+  }
+  __syncthreads();
+  int run = a0 + incl - local + wpre[w];
+  for (int k = 0; k < per; k++) {
+    const int c = t * per + k;
+    if (c < ncell) {
+      const int v = cnt[c];
+      start[c] = run;
+      cnt[c] = run;                  // cursor
+      run += v;
+    }
+  }
+  if (t == 0) start[ncell] = a0 + count;
+  __syncthreads();
+  for (int k = t; k < count; k += 1024) {
+    const int c = cell_of[a0 + k];
+    const int slot = atomicAdd(&cnt[c], 1);
+    out[slot] = a0 + k;
+  }
+  if (!sorted) return;
+  __syncthreads();
+  for (int c = t; c < ncell; c += 1024) {     // insertion sort of each (short) cell
+    const int b = start[c], e = start[c + 1];
+    for (int k = b + 1; k < e; k++) {
+      const int v = out[k];
+      int q = k - 1;
+      while (q >= b && out[q] > v) {
+        out[q + 1] = out[q];
+        q--;
+      }
+      out[q + 1] = v;
+    }
+  }
+  __syncthreads();
+}
+
+// Block index remapped so that each XCD (blocks are dealt to the 8 XCDs in
+// turn) works on one contiguous share of the index range, which keeps the
+// data of neighbouring blocks in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int x = b & 7, l = b >> 3, q = nb >> 3, rem = nb & 7;
+  return x * q + min(x, rem) + l;
+}
+
 // -------------------------------------------------- fixed-point scatter add
 // Each contribution is rounded to 2^-32 once, so sums are exact and their
 // value does not depend on the order in which waves arrive.

@@ -166,6 +166,7 @@ struct Model {
 // Device arena, work items and launch shapes of one or more systems that
 // are evaluated together.
 struct Pack {
+  unsigned long long uid = 0;          // distinct for every pack built in the process
   int n = 0, nh = 0, nsys = 0, ntypes = 0;
   int options = 0;
   int cap = 128;                       // neighbour-list row capacity (64, 128, 256)
@@ -207,8 +208,13 @@ struct SgHost {
   // training set
   int cap = 0, count = 0;
   float *F = nullptr, *target = nullptr;          // raw [cap][66], [cap]
-  float *X = nullptr;                             // normalised [cap][96]
-  float *y = nullptr;                             // normalised targets [cap]
+  std::vector<unsigned long long> keys;           // per row: (pack uid << 32) | atom of the pack
+  float *X = nullptr;                             // training rows, normalised [ntrain][96]
+  float *y = nullptr;                             // normalised targets [ntrain]
+  std::vector<int> order;                         // dataset row of each training row
+  int ntrain = 0;
+  float w_abs = 1.f, w_diff = 1.f;                // loss weights (w_diff per pair-difference variance)
+  float dscale = 0.f;                             // 1 / mean squared normalised pair difference
   int bcap = 0;
   SgTrain tb{};
   std::vector<void *> allocs, tallocs;
@@ -545,6 +551,8 @@ int build_items(Pack &P) {
 
 // Pack the systems `ms` (internal order each) into one device arena.
 int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
+  static unsigned long long next_uid = 1;
+  P.uid = next_uid++;
   P.nsys = (int)ms.size();
   P.options = options;
   // radius types: union over the systems, by exact equality
@@ -1750,14 +1758,22 @@ static int pack_sg_attach(Pack &P, int model, int every) {
     std::vector<int> flags(P.nsys, 1);
     std::vector<int2> rows;
     for (const SysInfo &S : P.sinfo)
-      for (int a = S.a0; a < S.a0 + S.n; a += 128) rows.push_back(make_int2(a, std::min(a + 128, S.a0 + S.n)));
+      for (int a = S.a0; a < S.a0 + S.n; a += kSgBlk) rows.push_back(make_int2(a, std::min(a + kSgBlk, S.a0 + S.n)));
     auto &L = P.sg_allocs;
-    if (upload(L, &d.sys_sg, flags) || upload(L, &d.sg_rows, rows) || dalloc(L, &d.sg_state, 4 * (size_t)P.nsys) ||
-        dalloc(L, &d.sg_F, 64 * nn) || dalloc(L, &d.sg_X, (size_t)kSgKPad * nn) || dalloc(L, &d.sg_s, nn) ||
-        dalloc(L, &d.sg_dsdF, 64 * nn) || dalloc(L, &d.sg_target, nn) || dalloc(L, &d.sg_c, nn) ||
-        dalloc(L, &d.sg_delta, nn) || dalloc(L, &d.sg_err, (size_t)P.nsys))
+    const size_t ns = std::max(P.nsys, 1);
+    if (upload(L, &d.sys_sg, flags) || upload(L, &d.sg_rows, rows) || dalloc(L, &d.sg_state, 4 * ns) ||
+        dalloc(L, &d.sg_any_plain, 1) || dalloc(L, &d.sg_F, 64 * nn) || dalloc(L, &d.sg_X, (size_t)kSgKPad * nn) ||
+        dalloc(L, &d.sg_s, nn) || dalloc(L, &d.sg_dsdF, 64 * nn) || dalloc(L, &d.sg_target, nn) ||
+        dalloc(L, &d.sg_c, nn) || dalloc(L, &d.sg_delta, nn) || dalloc(L, &d.sg_err, ns) ||
+        dalloc(L, &d.sg_nbr, rows.size() * kSgNbrCap * kSgBlk) || dalloc(L, &d.sg_nn, nn) ||
+        dalloc(L, &d.sg_ref, nn) || dalloc(L, &d.sg_nbl, 4 * ns) || dalloc(L, &d.sg_grid_p, ns) ||
+        dalloc(L, &d.sg_cstart, ns * (kMaxCells + 1)) || dalloc(L, &d.sg_catoms, nn) || dalloc(L, &d.sg_ctmp, nn))
       return AGBNP_ERR;
     d.nsg_rows = (int)rows.size();
+    // the descriptor lists depend on the coordinates only: they outlive a
+    // change of model (the guard compares against the positions they were
+    // built from)
+    HIPCHK(hipMemset(d.sg_nbl, 0, sizeof(int) * 4 * ns));
   }
   HIPCHK(hipMemset(d.sg_state, 0, sizeof(int) * 4 * P.nsys));
   HIPCHK(hipMemset(d.sg_c, 0, sizeof(double) * std::max(P.n, 1)));
@@ -1812,6 +1828,33 @@ int agbnp3_batch_sg_stats(int batch, double *err, int *mode) {
   return pack_sg_stats(B->p, err, mode);
 }
 
+static int pack_sg_lists(Pack &P, int *builds, int *overflow) {
+  if (P.sg_allocs.empty()) {
+    errprint("agbnp3_sg_lists(): no surrogate was attached.\n");
+    return AGBNP_ERR;
+  }
+  HIPCHK(hipStreamSynchronize(P.stream));
+  std::vector<int> st(4 * P.nsys);
+  HIPCHK(hipMemcpy(st.data(), P.d.sg_nbl, sizeof(int) * st.size(), hipMemcpyDeviceToHost));
+  for (int b = 0; b < P.nsys; b++) {
+    if (builds) builds[b] = st[4 * b + 2];
+    if (overflow) overflow[b] = st[4 * b + 3];
+  }
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_lists(int tag, int *builds, int *overflow) {
+  Instance *I = get_instance(tag, "agbnp3_sg_lists()");
+  if (!I) return AGBNP_ERR;
+  return pack_sg_lists(I->p, builds, overflow);
+}
+
+int agbnp3_batch_sg_lists(int batch, int *builds, int *overflow) {
+  Batch *B = get_batch(batch, "agbnp3_batch_sg_lists()");
+  if (!B) return AGBNP_ERR;
+  return pack_sg_lists(B->p, builds, overflow);
+}
+
 // training set: plain evaluations of a batch at device coordinates x, y, z
 int agbnp3_sg_reserve(int model, int natoms) {
   SgHost *M = get_model(model, "agbnp3_sg_reserve()");
@@ -1851,10 +1894,21 @@ int agbnp3_batch_sg_collect(int batch, int model, const double *x, const double 
                         false);
   if (rc == AGBNP_OK) rc = pack_sync(P, "agbnp3_batch_sg_collect()");
   if (rc == AGBNP_OK) {
+    std::vector<int> nbl(4 * P.nsys);
+    HIPCHK(hipMemcpy(nbl.data(), P.d.sg_nbl, sizeof(int) * nbl.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < P.nsys && rc == AGBNP_OK; b++)
+      if (nbl[4 * b + 3]) {
+        errprint("agbnp3_batch_sg_collect(): system %d has atoms with more than %d neighbours within %.1f A.\n", b,
+                 kSgNbrCap, (double)(kSgCut + kSgSkin));
+        rc = AGBNP_ERR;
+      }
+  }
+  if (rc == AGBNP_OK) {
     launch_sg_append(P.d, M->F, M->target, M->count, P.stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(P.stream));
     M->count += P.n;
+    for (int a = 0; a < P.n; a++) M->keys.push_back((P.uid << 32) | (unsigned long long)a);
   }
   if (keep_on) pack_sg_attach(P, keep_model, keep_every);
   else pack_sg_attach(P, -1, 0);
@@ -1923,18 +1977,53 @@ int agbnp3_sg_fit(int model, float margin, unsigned long long seed) {
   }
   M->ymu = (float)ym;
   M->ysd = (float)std::max(sqrt(ys / N), 1e-6);
-  std::vector<int> perm(N);
-  for (int i = 0; i < N; i++) perm[i] = i;
+  // pairs: rows of the same atom of the same pack (collected from different
+  // coordinates), taken in collection order two at a time; a row left over
+  // pairs with the one before it, an atom seen once with itself
+  std::vector<int> byk(N);
+  for (int i = 0; i < N; i++) byk[i] = i;
+  std::stable_sort(byk.begin(), byk.end(), [&](int a, int b) { return M->keys[a] < M->keys[b]; });
+  std::vector<std::pair<int, int>> pairs;
+  for (int g0 = 0; g0 < N;) {
+    int g1 = g0 + 1;
+    while (g1 < N && M->keys[byk[g1]] == M->keys[byk[g0]]) g1++;
+    const int m = g1 - g0;
+    for (int q = 0; q + 1 < m; q += 2) pairs.emplace_back(byk[g0 + q], byk[g0 + q + 1]);
+    if (m % 2) pairs.emplace_back(byk[g1 - 1], m > 1 ? byk[g1 - 2] : byk[g1 - 1]);
+    g0 = g1;
+  }
+  double dd = 0.0;
+  long long nd = 0;
+  for (const auto &pr : pairs)
+    if (pr.first != pr.second) {
+      const double e = (t[pr.first] - t[pr.second]) / M->ysd;
+      dd += e * e;
+      nd++;
+    }
+  M->dscale = nd && dd > 0.0 ? (float)(nd / dd) : 0.f;
   std::mt19937_64 rng(seed);
-  std::shuffle(perm.begin(), perm.end(), rng);
-  std::vector<float> X((size_t)N * kSgKPad, 0.f);
-  std::vector<float> y(N);
-  for (int r = 0; r < N; r++) {
-    const int i = perm[r];
+  if (nd) {
+    std::shuffle(pairs.begin(), pairs.end(), rng);
+    M->order.resize(2 * pairs.size());
+    for (size_t q = 0; q < pairs.size(); q++) {
+      M->order[2 * q] = pairs[q].first;
+      M->order[2 * q + 1] = pairs[q].second;
+    }
+  } else {                            // every atom seen once: plain shuffled rows
+    M->order.resize(N);
+    for (int i = 0; i < N; i++) M->order[i] = i;
+    std::shuffle(M->order.begin(), M->order.end(), rng);
+  }
+  const int NT = (int)M->order.size();
+  std::vector<float> X((size_t)NT * kSgKPad, 0.f);
+  std::vector<float> y(NT);
+  for (int r = 0; r < NT; r++) {
+    const int i = M->order[r];
     for (int k = 0; k < kSgNFeat; k++)
       X[(size_t)r * kSgKPad + k] = (F[(size_t)i * kSgNFeat + k] - norm[k]) * norm[kSgNFeat + k];
     y[r] = (float)((t[i] - ym) / M->ysd);
   }
+  M->ntrain = NT;
   HIPCHK(hipStreamSynchronize(M->stream));
   for (void *q : M->tallocs) (void)hipFree(q);
   M->tallocs.clear();
@@ -1962,10 +2051,12 @@ static int sg_batch_buffers(SgHost *M, int B) {
 // gradients of one batch of the shuffled training set (no update): tests
 int agbnp3_sg_grads(int model, int r0, int B, float *grad, float *loss) {
   SgHost *M = get_model(model, "agbnp3_sg_grads()");
-  if (!M || !M->X || B <= 0 || r0 < 0 || r0 + B > M->count) return AGBNP_ERR;
+  if (!M || !M->X || B <= 0 || r0 < 0 || r0 % 2 || r0 + B > M->ntrain) return AGBNP_ERR;
   if (sg_batch_buffers(M, B) != AGBNP_OK) return AGBNP_ERR;
   M->tb.X = M->X;
   M->tb.y = M->y;
+  M->tb.w_abs = M->w_abs;
+  M->tb.w_diff = M->w_diff * M->dscale;
   launch_sg_train_step(M->view(), M->tb, r0, B, M->g, M->loss, 256, M->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(M->stream));
@@ -1979,14 +2070,16 @@ int agbnp3_sg_grads(int model, int r0, int B, float *grad, float *loss) {
 int agbnp3_sg_train(int model, int epochs, int B, float lr, float decay, unsigned long long seed, float *epoch_loss) {
   SgHost *M = get_model(model, "agbnp3_sg_train()");
   if (!M || !M->X || B <= 0 || epochs <= 0) return AGBNP_ERR;
-  const int nb = M->count / B;
-  if (nb <= 0 || nb > (1 << 16)) {
-    errprint("agbnp3_sg_train(): %d rows do not make batches of %d.\n", M->count, B);
+  const int nb = M->ntrain / B;
+  if (nb <= 0 || nb > (1 << 16) || B % 2) {
+    errprint("agbnp3_sg_train(): %d rows do not make batches of %d (an even number).\n", M->ntrain, B);
     return AGBNP_ERR;
   }
   if (sg_batch_buffers(M, B) != AGBNP_OK) return AGBNP_ERR;
   M->tb.X = M->X;
   M->tb.y = M->y;
+  M->tb.w_abs = M->w_abs;
+  M->tb.w_diff = M->w_diff * M->dscale;
   int dev = 0, cu = 256;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) cu = prop.multiProcessorCount;
@@ -2009,6 +2102,23 @@ int agbnp3_sg_train(int model, int epochs, int B, float lr, float decay, unsigne
     for (float l : losses) s += l;
     if (epoch_loss) epoch_loss[e] = (float)(s / nb);
   }
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_set_loss(int model, float w_abs, float w_diff) {
+  SgHost *M = get_model(model, "agbnp3_sg_set_loss()");
+  if (!M || !(w_abs >= 0.f) || !(w_diff >= 0.f)) return AGBNP_ERR;
+  M->w_abs = w_abs;
+  M->w_diff = w_diff;
+  return AGBNP_OK;
+}
+
+int agbnp3_sg_train_rows(int model, int *count, int *rows, float *diff_scale) {
+  SgHost *M = get_model(model, "agbnp3_sg_train_rows()");
+  if (!M) return AGBNP_ERR;
+  if (count) *count = M->ntrain;
+  if (rows) std::copy(M->order.begin(), M->order.end(), rows);
+  if (diff_scale) *diff_scale = M->dscale;
   return AGBNP_OK;
 }
 

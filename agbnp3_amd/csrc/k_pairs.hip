@@ -43,6 +43,7 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
   __shared__ float4 cpos[kTile];     // x, y, z, 1/R
   __shared__ float2 cw[kTile];       // sp/R, rtype (as float bits)
   __shared__ int slot;
+  if (d.sg_on && !*d.sg_any_plain) return;   // every system on the surrogate this call (grid-uniform)
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
   int staged = -1;
   for (int it = next_item(&d.queue[Q_DESC], &slot); it < d.ndesc_items; it = next_item(&d.queue[Q_DESC], &slot)) {
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
 void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s) {
   if (!d.ndesc_items) return;
   const size_t lds = (size_t)d.ntypes * kTblN * sizeof(float2);
-  const int grid = t.desc_grid > 0 ? std::min(t.desc_grid, d.ndesc_items) : d.ndesc_items;
+  int grid = t.desc_grid > 0 ? std::min(t.desc_grid, d.ndesc_items) : d.ndesc_items;
+  if (d.sg_on) grid = std::min(grid, kSgPersistentGrid);   // calls where every block leaves at once
   if (d.rows == 64) hipLaunchKernelGGL(k_descreen<64>, dim3(grid), dim3(64), lds, s, d);
   else if (d.rows == 128) hipLaunchKernelGGL(k_descreen<128>, dim3(grid), dim3(128), lds, s, d);
   else hipLaunchKernelGGL(k_descreen<256>, dim3(grid), dim3(256), lds, s, d);
@@ -223,6 +225,7 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
   const float sgn2 = (d.options & 2) ? -1.f : 1.f;
+  if (d.sg_on && !*d.sg_any_plain) return;   // every system on the surrogate this call (grid-uniform)
   for (int it = next_item(&d.queue[Q_CHAIN], &slot); it < d.nchain_items;
        it = next_item(&d.queue[Q_CHAIN], &slot)) {
     const int4 item = d.chain_items[it];
@@ -331,7 +334,8 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
 
 void launch_chain(const Dev &d, const Tuning &t, hipStream_t s) {
   if (!d.nchain_items) return;
-  const int grid = t.chain_grid > 0 ? std::min(t.chain_grid, d.nchain_items) : d.nchain_items;
+  int grid = t.chain_grid > 0 ? std::min(t.chain_grid, d.nchain_items) : d.nchain_items;
+  if (d.sg_on) grid = std::min(grid, kSgPersistentGrid);
   hipLaunchKernelGGL(k_chain, dim3(grid), dim3(256), 0, s, d);
 }
 

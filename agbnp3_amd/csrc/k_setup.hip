@@ -19,6 +19,7 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
                                                 const double *__restrict__ z) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (d.sg_on && i < d.nsys) d.sg_state[4 * i + 3] = 0;   // surrogate: out-of-range flag of the call
+  if (d.sg_on && i == 0) *d.sg_any_plain = 0;
   if (i < d.n) {
     const int e = d.int2ext[i];
     d.pos[i] = make_float4((float)x[e], (float)y[e], (float)z[e], d.R[i]);
@@ -67,98 +68,18 @@ __global__ __launch_bounds__(1024) void k_cells(Dev d) {
   __shared__ int wsum[16], wpre[16];
   __shared__ float box[6][16];
   __shared__ GridParams g;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const SysInfo S = d.sys[blockIdx.x];
-  const int a0 = S.a0, nh = S.nh;
-  int *cstart = d.cell_start + (size_t)blockIdx.x * (kMaxCells + 1);
-  {
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int k = t; k < nh; k += 1024) {
-      const float4 p = d.pos[a0 + k];
-      lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
-      hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const float a = wave_min(lo[k]), b = wave_max(hi[k]);
-      if (lane == 0) { box[k][w] = a; box[3 + k][w] = b; }
-    }
-  }
-  __syncthreads();
-  if (t == 0) {
-    float bl[3], bh[3];
-    for (int k = 0; k < 3; k++) {
-      bl[k] = box[k][0];
-      bh[k] = box[3 + k][0];
-      for (int q = 1; q < 16; q++) { bl[k] = fminf(bl[k], box[k][q]); bh[k] = fmaxf(bh[k], box[3 + k][q]); }
-    }
-    float cs = fmaxf(kNbOffset * 2.0f * S.rmax_heavy, kNbOffset * (S.rmax_heavy + kHbRadius)) * 1.001f + 1e-3f;
-    float ext[3];
-    for (int k = 0; k < 3; k++) {
-      float lo = bl[k], hi = bh[k];
-      if (!(hi >= lo)) { lo = 0.f; hi = 0.f; }    // no heavy atoms
-      g.lo[k] = lo - 1e-3f;
-      ext[k] = hi - lo + 2e-3f;
-      if (!(ext[k] < 1e7f)) ext[k] = 1e7f;          // non-finite input: keep the loop finite
-    }
-    for (;;) {
-      long long nc = 1;
-      for (int k = 0; k < 3; k++) {
-        g.dim[k] = (int)floorf(ext[k] / cs) + 1;
-        nc *= g.dim[k];
-      }
-      if (nc <= kMaxCells) { g.ncell = (int)nc; break; }
-      cs *= 1.05f;
-    }
-    g.cs = cs;
-    g.inv_cs = 1.0f / cs;
+  float lo[3], hi[3];
+  block_bbox(d.pos, S.a0, S.nh, box, lo, hi);
+  if (threadIdx.x == 0) {
+    const float cs = fmaxf(kNbOffset * 2.0f * S.rmax_heavy, kNbOffset * (S.rmax_heavy + kHbRadius)) * 1.001f + 1e-3f;
+    g = make_grid(lo, hi, cs);
     d.grid[blockIdx.x] = g;
   }
   __syncthreads();
-  const int ncell = g.ncell;
-  for (int c = t; c < ncell; c += 1024) cnt[c] = 0;
-  __syncthreads();
-  for (int k = t; k < nh; k += 1024) {
-    const float4 p = d.pos[a0 + k];
-    const int c = cell_index(g, p.x, p.y, p.z);
-    d.cell_tmp[a0 + k] = c;
-    atomicAdd(&cnt[c], 1);
-  }
-  __syncthreads();
-  // exclusive scan of cnt over the cells, contiguous cells per thread
-  const int per = (ncell + 1023) / 1024;
-  int local = 0;
-  for (int k = 0; k < per; k++) {
-    const int c = t * per + k;
-    if (c < ncell) local += cnt[c];
-  }
-  const int incl = wave_incl_scan(local);
-  if (lane == 63) wsum[w] = incl;
-  __syncthreads();
-  if (t < 16) {
-    int s = 0;
-    for (int q = 0; q < t; q++) s += wsum[q];
-    wpre[t] = s;
-  }
-  __syncthreads();
-  int run = a0 + incl - local + wpre[w];
-  for (int k = 0; k < per; k++) {
-    const int c = t * per + k;
-    if (c < ncell) {
-      const int v = cnt[c];
-      cstart[c] = run;
-      cnt[c] = run;                  // cursor
-      run += v;
-    }
-  }
-  if (t == 0) cstart[ncell] = a0 + nh;
-  __syncthreads();
-  // scatter; the order inside a cell is irrelevant (rows are re-sorted)
-  for (int k = t; k < nh; k += 1024) {
-    const int c = d.cell_tmp[a0 + k];
-    const int slot = atomicAdd(&cnt[c], 1);
-    d.cell_atoms[slot] = a0 + k;
-  }
+  // the order inside a cell is irrelevant (rows are re-sorted)
+  bin_atoms(g, d.pos, S.a0, S.nh, d.cell_tmp, d.cell_start + (size_t)blockIdx.x * (kMaxCells + 1), d.cell_atoms, cnt,
+            wsum, wpre, false);
 }
 
 void launch_cells(const Dev &d, hipStream_t s) {

@@ -6,10 +6,14 @@
 // between plain anchor calls.
 //
 // Descriptors (per atom i): four heavy-atom channels (by R) x 16 radial
-// shells up to 12 A, F[c][k] = sum_j B3((d_ij - mu_k)/h) T(d_ij) with
-// h = 0.75 A, mu_k = h (k + 1/2), B3 the cubic B-spline and T a cosine taper
-// on [10.5, 12] A; then R_i and q_i.  Normalised (FP32) they are the MLP
-// input, padded to 96 columns.
+// shells up to 12 A, F[c][k] = sum_j g_j B3((d_ij - mu_k)/h) T(d_ij) with
+// h = 0.75 A, mu_k = h (k + 1/2), B3 the cubic B-spline and T a quintic
+// smoothstep taper on [10.5, 12] A; then R_i and q_i.  Each neighbour j counts with its
+// self-volume fraction g_j = V'_j / V_j (the factor the descreening applies
+// to it, computed by the overlap tree on every call), so the shell sums carry
+// what burial does to each descreener; the gradient reaches g_j through the
+// tree derivative pass like the plain descreening's.  Normalised (FP32) they
+// are the MLP input, padded to 96 columns.
 //
 // MLP: 66 -> 128 -> 128 -> 128 -> 1, SiLU, on the matrix cores
 // (v_mfma_f32_16x16x32_bf16) with FP32 accumulation and biases.  Weights are
@@ -31,7 +35,7 @@ namespace agb {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr float kSgH = 0.75f, kSgCut = 12.0f, kSgTaper0 = 10.5f;
+constexpr float kSgH = 0.75f, kSgTaper0 = 10.5f;
 constexpr int kSgTile = 32;                 // atoms per MLP tile
 constexpr int kLd1 = kSgKPad + 8;           // LDS row strides (bf16 elements), padded
 constexpr int kLdH = kSgHid + 8;
@@ -42,94 +46,212 @@ __device__ __forceinline__ float silup(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-// cubic B-spline and its derivative
-__device__ __forceinline__ float bsp3(float t) {
-  const float a = fabsf(t);
-  if (a < 1.f) return 2.f / 3.f - a * a + 0.5f * a * a * a;
-  if (a < 2.f) { const float u = 2.f - a; return u * u * u * (1.f / 6.f); }
-  return 0.f;
+// The four cubic B-splines that are non-zero at u = d/h - 1/2 belong to
+// shells k0 .. k0 + 3 with k0 = floor(u) - 1; with f = u - floor(u) they are
+// the uniform B-spline blending polynomials (1-f)^3/6, (3f^3 - 6f^2 + 4)/6,
+// (-3f^3 + 3f^2 + 3f + 1)/6, f^3/6.  Returns k0; b and db (derivatives in d).
+__device__ __forceinline__ int sg_basis(float dist, float b[4], float db[4]) {
+  const float u = dist * (1.f / kSgH) - 0.5f;
+  const float fl = floorf(u), f = u - fl, g = 1.f - f, f2 = f * f, g2 = g * g;
+  constexpr float s6 = 1.f / 6.f, ih = 1.f / kSgH;
+  b[0] = g2 * g * s6;
+  b[1] = (2.f / 3.f) - f2 + 0.5f * f2 * f;
+  b[3] = f2 * f * s6;
+  b[2] = 1.f - b[0] - b[1] - b[3];              // partition of unity
+  db[0] = -0.5f * g2 * ih;
+  db[1] = (1.5f * f2 - 2.f * f) * ih;
+  db[3] = 0.5f * f2 * ih;
+  db[2] = -(db[0] + db[1] + db[3]);
+  return (int)fl - 1;
 }
-__device__ __forceinline__ float bsp3p(float t) {
-  const float a = fabsf(t);
-  float v = 0.f;
-  if (a < 1.f) v = -2.f * a + 1.5f * a * a;
-  else if (a < 2.f) { const float u = 2.f - a; v = -0.5f * u * u; }
-  return t < 0.f ? -v : v;
-}
+// 1 up to kSgTaper0, 0 from kSgCut, the quintic smoothstep between (C2)
 __device__ __forceinline__ void sg_taper(float d, float &T, float &Tp) {
-  if (d <= kSgTaper0) { T = 1.f; Tp = 0.f; return; }
-  if (d >= kSgCut) { T = 0.f; Tp = 0.f; return; }
-  const float w = kSgCut - kSgTaper0, x = (d - kSgTaper0) / w;
-  T = 0.5f * (1.f + cospif(x));
-  Tp = -0.5f * kPiF / w * sinpif(x);
+  constexpr float iw = 1.f / (kSgCut - kSgTaper0);
+  const float x = fminf(fmaxf((d - kSgTaper0) * iw, 0.f), 1.f), x2 = x * x, y = 1.f - x;
+  T = 1.f - x2 * x * (10.f - 15.f * x + 6.f * x2);
+  Tp = -30.f * x2 * y * y * iw;
 }
 __device__ __forceinline__ int sg_channel(float R) {
   return R < 2.035f ? 0 : (R < 2.125f ? 1 : (R < 2.25f ? 2 : 3));
 }
 
-// ------------------------------------------------------------ descriptors
-// item = a block of up to kSgRows consecutive atoms of one system against
-// that system's heavy atoms; each thread owns one row's 64 shell sums in LDS
-constexpr int kSgRows = 128;
+// ------------------------------------------------ descriptor neighbour lists
+// Every pair (i, j heavy) within kSgCut enters the descriptors and their
+// gradient.  Per system the pairs within kSgCut + kSgSkin are listed once and
+// kept while no atom has moved more than kSgSkin / 2 since (then every pair
+// now within kSgCut was within kSgCut + kSgSkin at the build).  Lists come
+// from an all-atom cell grid, scanned in cell order with each cell in atom
+// order, so their order (and every FP32 sum over them) is reproducible.
+constexpr float kSgCellEdge = 4.5f;
 
-__global__ __launch_bounds__(kSgRows) void k_sg_features(Dev d) {
-  __shared__ float F[kSgRows][65];
-  __shared__ float4 cpos[256];     // x, y, z, channel
-  const int2 blk = d.sg_rows[blockIdx.x];
-  const int sys = d.asys[blk.x];
-  if (!d.sys_sg[sys]) return;       // no model on this system (block-uniform)
-  const SysInfo S = d.sys[sys];
-  const int i = blk.x + threadIdx.x;
-  const bool row = i < blk.y;
-  for (int k = 0; k < 64; k++) F[threadIdx.x][k] = 0.f;
-  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (row) pi = d.pos[i];
-  for (int base = S.a0; base < S.a0 + S.nh; base += 256) {
-    __syncthreads();
-    for (int k = threadIdx.x; k < 256; k += kSgRows) {
-      const int j = base + k;
-      if (j < S.a0 + S.nh) {
-        const float4 p = d.pos[j];
-        cpos[k] = make_float4(p.x, p.y, p.z, (float)sg_channel(p.w));
-      }
+// per system: rebuild on the first call and after a move beyond kSgSkin / 2
+__global__ __launch_bounds__(256) void k_sg_guard(Dev d) {
+  const int b = blockIdx.x;
+  if (!d.sys_sg[b]) return;                         // block-uniform
+  int *st = d.sg_nbl + 4 * b;
+  const SysInfo S = d.sys[b];
+  const bool built = st[0] != 0;
+  const float lim2 = 0.25f * kSgSkin * kSgSkin;
+  int moved = !built;
+  if (built)
+    for (int i = S.a0 + threadIdx.x; i < S.a0 + S.n; i += 256) {
+      const float4 p = d.pos[i], r = d.sg_ref[i];
+      const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
+      moved |= !(ex * ex + ey * ey + ez * ez <= lim2);   // non-finite counts as moved
     }
-    __syncthreads();
-    const int nc = min(256, S.a0 + S.nh - base);
-    if (row) {
-      for (int c = 0; c < nc; c++) {
-        const float4 pj = cpos[c];
-        const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-        const float d2 = ex * ex + ey * ey + ez * ez;
-        if (!(d2 < kSgCut * kSgCut) || base + c == i) continue;
-        const float dist = sqrtf(d2);
-        float T, Tp;
-        sg_taper(dist, T, Tp);
-        const float u = dist * (1.f / kSgH) - 0.5f;
-        const int k0 = (int)floorf(u) - 1;
-        float *f = &F[threadIdx.x][(int)pj.w * 16];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int k = k0 + q;
-          if (k >= 0 && k < kSgNShell) f[k] += bsp3(u - (float)k) * T;
+  moved = __syncthreads_or(moved);
+  if (threadIdx.x == 0) {
+    st[1] = moved;
+    if (moved) st[2] += 1;
+  }
+}
+
+// all atoms of a rebuilt system into cells of edge >= kSgCellEdge
+__global__ __launch_bounds__(1024) void k_sg_cells(Dev d) {
+  __shared__ int cnt[kMaxCells];
+  __shared__ int wsum[16], wpre[16];
+  __shared__ float box[6][16];
+  __shared__ GridParams g;
+  const int b = blockIdx.x;
+  if (!d.sys_sg[b] || !d.sg_nbl[4 * b + 1]) return;   // block-uniform
+  const SysInfo S = d.sys[b];
+  float lo[3], hi[3];
+  block_bbox(d.pos, S.a0, S.n, box, lo, hi);
+  if (threadIdx.x == 0) {
+    g = make_grid(lo, hi, kSgCellEdge);
+    d.sg_grid_p[b] = g;
+  }
+  __syncthreads();
+  bin_atoms(g, d.pos, S.a0, S.n, d.sg_ctmp, d.sg_cstart + (size_t)b * (kMaxCells + 1), d.sg_catoms, cnt, wsum, wpre,
+            true);
+}
+
+// one wavefront per row (four per block, over one row block): the cells
+// within reach, row by row of cells (contiguous in the sorted atoms), 64
+// candidates at a time, compacted by ballot
+__global__ __launch_bounds__(256) void k_sg_nbl(Dev d) {
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int2 blk = d.sg_rows[rb];
+  const int sys = d.asys[blk.x];
+  if (!d.sys_sg[sys] || !d.sg_nbl[4 * sys + 1]) return;   // block-uniform
+  const SysInfo S = d.sys[sys];
+  const GridParams g = d.sg_grid_p[sys];
+  const int *cstart = d.sg_cstart + (size_t)sys * (kMaxCells + 1);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const float R = kSgCut + kSgSkin, R2 = R * R;
+  const int hend = S.a0 + S.nh;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int r = w; r < blk.y - blk.x; r += 4) {
+    const int a = d.sg_catoms[blk.x + r];          // rows in cell order: compact blocks
+    int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
+    const float4 pa = d.pos[a];
+    const bool aheavy = a < hend;
+    int lo[3], hi[3];
+    cell_coords(g, pa.x - R, pa.y - R, pa.z - R, lo[0], lo[1], lo[2]);
+    cell_coords(g, pa.x + R, pa.y + R, pa.z + R, hi[0], hi[1], hi[2]);
+    int nh = 0, nH = 0;                              // wave-uniform counts
+    for (int cz = lo[2]; cz <= hi[2]; cz++)
+      for (int cy = lo[1]; cy <= hi[1]; cy++) {
+        const int c0 = (cz * g.dim[1] + cy) * g.dim[0];
+        const int beg = cstart[c0 + lo[0]], end = cstart[c0 + hi[0] + 1];
+        for (int k0 = beg; k0 < end; k0 += 64) {
+          const int k = k0 + lane;
+          int j = -1;
+          bool th = false, tH = false;
+          if (k < end) {
+            j = d.sg_catoms[k];
+            const float4 pj = d.pos[j];
+            const float ex = pj.x - pa.x, ey = pj.y - pa.y, ez = pj.z - pa.z;
+            const bool in = ex * ex + ey * ey + ez * ez < R2 && j != a;
+            th = in && j < hend;
+            tH = in && j >= hend && aheavy;
+          }
+          const unsigned long long mh = __ballot(th), mH = __ballot(tH);
+          if (th) {
+            const int slot = nh + __popcll(mh & below);
+            if (slot < kSgNbrCap) nb[slot] = j;
+          }
+          if (tH) {
+            const int slot = kSgNbrCap - 1 - (nH + __popcll(mH & below));
+            if (slot >= 0) nb[slot] = j;
+          }
+          nh += __popcll(mh);
+          nH += __popcll(mH);
         }
       }
+    if (lane == 0) {
+      d.sg_ref[a] = pa;
+      const bool over = nh + nH > kSgNbrCap;
+      d.sg_nn[a] = over ? make_int2(0, 0) : make_int2(nh, nH);
+      if (over) d.sg_nbl[4 * sys + 3] = 1;          // the system stays on the plain kernels
     }
   }
-  if (!row) return;
-  // raw shell sums, normalised bf16 input row, in-range test
-  bool oor = false;
-  float *x = d.sg_X + (size_t)i * kSgKPad;
-  for (int k = 0; k < kSgKPad; k++) {
-    float v = 0.f;
-    if (k < kSgNFeat) {
-      const float raw = k < 64 ? F[threadIdx.x][k] : (k == 64 ? pi.w : d.q[i]);
-      if (k < 64) d.sg_F[(size_t)i * 64 + k] = raw;
-      v = (raw - d.sg.xmu[k]) * d.sg.xsd_inv[k];
-      oor |= !(v >= d.sg.xlo[k] && v <= d.sg.xhi[k]);
+  if (threadIdx.x == 0) d.sg_nbl[4 * sys + 0] = 1;
+}
+
+// ------------------------------------------------------------ descriptors
+// one wavefront per row, its lanes over the row's heavy neighbours; the 64
+// shell sums of the row are LDS bins of the wave (each pair adds to the 4
+// shells its B-spline covers, LDS float atomics), then lane m writes bin m
+__global__ __launch_bounds__(256) void k_sg_features(Dev d) {
+  __shared__ float bins[4][64];
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int2 blk = d.sg_rows[rb];
+  const int sys = d.asys[blk.x];
+  if (!d.sys_sg[sys]) return;       // no model on this system (block-uniform)
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool over = d.sg_nbl[4 * sys + 3] != 0;
+  float *bin = bins[w];
+  for (int r = w; r < blk.y - blk.x; r += 4) {
+    const int i = d.sg_catoms[blk.x + r];
+    const float4 pi = d.pos[i];
+    const int nh = over ? 0 : d.sg_nn[i].x;
+    const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
+    bin[lane] = 0.f;
+    // each lane takes its own stretch of the list (neighbouring entries are
+    // near each other and would add to the same bins at the same time)
+    const int chunk = (nh + 63) >> 6;
+    for (int st = 0; st < chunk; st++) {
+      const int k = lane * chunk + st;
+      if (k >= nh) continue;
+      const int j = nb[k];
+      const float4 pj = d.pos[j];
+      const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+      const float d2 = ex * ex + ey * ey + ez * ez;
+      if (!(d2 < kSgCut * kSgCut)) continue;
+      const float dist = d2 * rsqrtf(d2);
+      float T, Tp, b[4], db[4];
+      sg_taper(dist, T, Tp);
+      T *= d.sp_f[j];
+      const int k0s = sg_basis(dist, b, db);
+      float *f = bin + sg_channel(pj.w) * 16;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int kk = k0s + q;
+        if (kk >= 0 && kk < kSgNShell) atomicAdd(&f[kk], b[q] * T);
+      }
     }
-    x[k] = v;
+    // raw shell sums, normalised input row, in-range test
+    const float raw = bin[lane];
+    d.sg_F[(size_t)i * 64 + lane] = raw;
+    float v = (raw - d.sg.xmu[lane]) * d.sg.xsd_inv[lane];
+    bool oor = !(v >= d.sg.xlo[lane] && v <= d.sg.xhi[lane]);
+    float *x = d.sg_X + (size_t)i * kSgKPad;
+    x[lane] = v;
+    if (lane < kSgKPad - 64) {                    // R, q, then zero padding
+      const int k = 64 + lane;
+      float u = 0.f;
+      if (k < kSgNFeat) {
+        const float rw = k == 64 ? pi.w : d.q[i];
+        u = (rw - d.sg.xmu[k]) * d.sg.xsd_inv[k];
+        oor |= !(u >= d.sg.xlo[k] && u <= d.sg.xhi[k]);
+      }
+      x[k] = u;
+    }
+    if (__any(oor || over)) {
+      if (lane == 0) d.sg_state[4 * sys + 3] = 1;
+    }
   }
-  if (oor) d.sg_state[4 * sys + 3] = 1;
 }
 
 // ---------------------------------------------------------------- the MLP
@@ -308,8 +430,10 @@ __global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
 
 // ---------------------------------------------------------------- training
 // One step over a batch of B rows of the normalised training set: forward,
-// d(mean squared error)/dy, the deltas of the three hidden layers, stored
-// feature-major (transposed) for the weight-gradient products.
+// d(loss)/dy, the deltas of the three hidden layers, stored feature-major
+// (transposed) for the weight-gradient products.  The loss per row is
+// w_abs e^2 + w_diff (e - e_p)^2 with e = y - t and p the row's pair partner
+// (rows 2k, 2k+1; the batch starts on an even row).
 __global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r0, int B) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
@@ -319,13 +443,16 @@ __global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r
     const int b0 = tile * kSgTile, nrows = min(kSgTile, B - b0);
     __syncthreads();
     const float y = mlp_forward_tile(m, L, t.X, r0 + b0, nrows);
+    // error, and the partner row's (same wave: rows 2k, 2k+1 of an even start)
+    const float e = threadIdx.x < nrows ? y - t.y[r0 + b0 + threadIdx.x] : 0.f;
+    const float ep = __shfl_xor(e, 1, 64);
     if (threadIdx.x < kSgTile) {
       float dy = 0.f;
       if (threadIdx.x < nrows) {
-        const float e = y - t.y[r0 + b0 + threadIdx.x];
-        dy = 2.0f * e / (float)B;
+        const float de = (threadIdx.x ^ 1) < nrows ? e - ep : 0.f;
+        dy = (2.0f * t.w_abs * e + 4.0f * t.w_diff * de) / (float)B;
         t.dy[b0 + threadIdx.x] = dy;
-        t.err2[b0 + threadIdx.x] = e * e;
+        t.err2[b0 + threadIdx.x] = t.w_abs * e * e + t.w_diff * de * de;
       }
       L.seed()[threadIdx.x] = dy;
     }
@@ -466,95 +593,139 @@ __global__ void k_sg_mode(Dev d) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= d.nsys) return;
   int *st = d.sg_state + 4 * b;    // [0] calls, [1] anchored, [2] mode (1 = surrogate), [3] out of range
-  if (!d.sys_sg[b]) { st[2] = 0; return; }
+  if (!d.sys_sg[b]) {
+    st[2] = 0;
+    *d.sg_any_plain = 1;
+    return;
+  }
   const int calls = st[0] + 1;
   st[0] = calls;
-  const bool anchor = !st[1] || st[3] || d.sg_every <= 1 || (calls - 1) % d.sg_every == 0;
+  const bool anchor = !st[1] || st[3] || d.sg_nbl[4 * b + 3] || d.sg_every <= 1 || (calls - 1) % d.sg_every == 0;
   st[2] = anchor ? 0 : 1;
+  if (anchor) *d.sg_any_plain = 1;
 }
 
 // ------------------------------------------------------- gradient pass
-// For surrogate-mode systems: row a collects -w d s_a / d x_a (a as centre,
-// heavy neighbours i) and -w_i d s_i / d x_a (a heavy, as neighbour of every
-// atom i), for w = q2ab (GB part, acc_ch 0..2) and w = abrw (vdW, 3..5).
-// d s_i / d x_a = sum_k dsdF_i[ch(a)][k] phi_k'(d) (x_a - x_i) / d.
-constexpr int kSgCols = 64;
-
-__global__ __launch_bounds__(kSgRows) void k_sg_grad(Dev d) {
-  __shared__ float G[kSgRows][65];          // own ds/dF
-  __shared__ float Gc[kSgCols][65];         // columns' ds/dF
-  __shared__ float4 cpos[kSgCols];          // x, y, z, channel (-1: hydrogen)
-  __shared__ float2 cw[kSgCols];            // q2ab, abrw
-  const int2 blk = d.sg_rows[blockIdx.x];
+// For surrogate-mode systems, one wavefront per row a, lanes over its list:
+//   a as centre, heavy neighbours j:      -w_a d s_a / d x_a
+//   a heavy, as neighbour of each atom i: -w_i d s_i / d x_a
+// for w = q2ab (GB part, acc_ch 0..2) and w = abrw (vdW, 3..5), where
+// d s_i / d x_a = g_a sum_k dsdF_i[ch(a)][k] phi_k'(d) (x_a - x_i) / d, and
+// the self-volume path sum_i w_i d s_i / d g_a = sum_i w_i sum_k
+// dsdF_i[ch(a)][k] phi_k(d) into acc_ch 6, 7 (the tree derivative pass).
+__global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
+  __shared__ float Gs[4][64];               // the row's own ds/dF, per wave
+  const int rb = xcd_block(blockIdx.x, gridDim.x);
+  const int2 blk = d.sg_rows[rb];
   const int sys = d.asys[blk.x];
   if (d.sg_state[4 * sys + 2] != 1) return;  // plain call for this system (block-uniform)
   const SysInfo S = d.sys[sys];
-  const int a = blk.x + threadIdx.x;
-  const bool row = a < blk.y;
-  const bool aheavy = row && a < S.a0 + S.nh;
-  float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (row) {
-    pa = d.pos[a];
-    for (int k = 0; k < 64; k++) G[threadIdx.x][k] = d.sg_dsdF[(size_t)a * 64 + k];
-  }
-  const int cha = aheavy ? sg_channel(pa.w) : 0;
-  float sx = 0.f, sy = 0.f, sz = 0.f;                 // d s_a / d x_a
-  float gx = 0.f, gy = 0.f, gz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;   // neighbour terms
-  for (int base = S.a0; base < S.a0 + S.n; base += kSgCols) {
-    __syncthreads();
-    const int nc = min(kSgCols, S.a0 + S.n - base);
-    for (int k = threadIdx.x; k < nc * 64; k += kSgRows) {
-      const int c = k >> 6, f = k & 63;
-      Gc[c][f] = d.sg_dsdF[(size_t)(base + c) * 64 + f];
-    }
-    if (threadIdx.x < nc) {
-      const int j = base + threadIdx.x;
-      const float4 p = d.pos[j];
-      cpos[threadIdx.x] = make_float4(p.x, p.y, p.z, j < S.a0 + S.nh ? (float)sg_channel(p.w) : -1.f);
-      cw[threadIdx.x] = make_float2(d.q2ab_f[j], d.abrw_f[j]);
-    }
-    __syncthreads();
-    if (!row) continue;
-    for (int c = 0; c < nc; c++) {
-      const float4 pj = cpos[c];
-      const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;   // x_a - x_i
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float *Ga = Gs[w];
+  for (int r = w; r < blk.y - blk.x; r += 4) {
+    const int a = d.sg_catoms[blk.x + r];
+    const float4 pa = d.pos[a];
+    const int2 nn = d.sg_nn[a];
+    const bool aheavy = a < S.a0 + S.nh;
+    const int cha = aheavy ? sg_channel(pa.w) : 0;
+    const float gama = aheavy ? d.sp_f[a] : 0.f;
+    const float *dsdF = d.sg_dsdF + cha * 16;
+    const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
+    Ga[lane] = d.sg_dsdF[(size_t)a * 64 + lane];
+    float sx = 0.f, sy = 0.f, sz = 0.f;                 // d s_a / d x_a
+    float gx = 0.f, gy = 0.f, gz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;   // neighbour terms
+    float ru = 0.f, rv = 0.f;                           // sum_i {q2ab_i, abrw_i} d s_i / d g_a
+    // heavy neighbours: both terms (the second for heavy rows)
+    for (int k0 = 0; k0 < nn.x; k0 += 64) {
+      const int k = k0 + lane;
+      if (k >= nn.x) continue;
+      const int j = nb[k];
+      const float4 pj = d.pos[j];
+      const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;   // x_a - x_j
       const float d2 = ex * ex + ey * ey + ez * ez;
-      if (!(d2 < kSgCut * kSgCut) || base + c == a) continue;
-      const bool iheavy = pj.w >= 0.f;
-      if (!iheavy && !aheavy) continue;
-      const float dist = sqrtf(d2), inv = 1.0f / dist;
-      float T, Tp;
+      if (!(d2 < kSgCut * kSgCut)) continue;
+      const float inv = rsqrtf(d2), dist = d2 * inv;
+      float T, Tp, b[4], db[4];
       sg_taper(dist, T, Tp);
-      const float u = dist * (1.f / kSgH) - 0.5f;
-      const int k0 = (int)floorf(u) - 1;
-      float cA = 0.f, cB = 0.f;
-      const int chi = iheavy ? (int)pj.w : 0;
+      const int ks = sg_basis(dist, b, db);
+      const float *Gc = Ga + sg_channel(pj.w) * 16;
+      const float *Gj = dsdF + (size_t)j * 64;
+      float cA = 0.f, cB = 0.f, cU = 0.f;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int k = k0 + q;
-        if (k < 0 || k >= kSgNShell) continue;
-        const float t = u - (float)k;
-        const float dphi = bsp3p(t) * (1.f / kSgH) * T + bsp3(t) * Tp;
-        if (iheavy) cA += G[threadIdx.x][chi * 16 + k] * dphi;
-        if (aheavy) cB += Gc[c][cha * 16 + k] * dphi;
+        const int kk = ks + q;
+        if (kk < 0 || kk >= kSgNShell) continue;
+        const float dphi = db[q] * T + b[q] * Tp;
+        cA += Gc[kk] * dphi;
+        if (aheavy) {
+          const float g = Gj[kk];
+          cB += g * dphi;
+          cU += g * b[q];
+        }
       }
-      cA *= inv;
-      cB *= inv;
+      cA *= d.sp_f[j] * inv;          // the neighbour's weight in the shell sums
       sx += cA * ex; sy += cA * ey; sz += cA * ez;
-      const float2 w = cw[c];
-      gx += w.x * cB * ex; gy += w.x * cB * ey; gz += w.x * cB * ez;
-      vx += w.y * cB * ex; vy += w.y * cB * ey; vz += w.y * cB * ez;
+      if (aheavy) {
+        const float wq = d.q2ab_f[j], wa = d.abrw_f[j];
+        cB *= gama * inv;
+        cU *= T;
+        gx += wq * cB * ex; gy += wq * cB * ey; gz += wq * cB * ez;
+        vx += wa * cB * ex; vy += wa * cB * ey; vz += wa * cB * ez;
+        ru += wq * cU;
+        rv += wa * cU;
+      }
+    }
+    // hydrogen neighbours of heavy rows: the second term only
+    for (int k0 = 0; k0 < nn.y; k0 += 64) {
+      const int k = k0 + lane;
+      if (k >= nn.y) continue;
+      const int j = nb[kSgNbrCap - 1 - k];
+      const float4 pj = d.pos[j];
+      const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;
+      const float d2 = ex * ex + ey * ey + ez * ez;
+      if (!(d2 < kSgCut * kSgCut)) continue;
+      const float inv = rsqrtf(d2), dist = d2 * inv;
+      float T, Tp, b[4], db[4];
+      sg_taper(dist, T, Tp);
+      const int ks = sg_basis(dist, b, db);
+      const float *Gj = dsdF + (size_t)j * 64;
+      float cB = 0.f, cU = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int kk = ks + q;
+        if (kk < 0 || kk >= kSgNShell) continue;
+        const float g = Gj[kk];
+        cB += g * (db[q] * T + b[q] * Tp);
+        cU += g * b[q];
+      }
+      const float wq = d.q2ab_f[j], wa = d.abrw_f[j];
+      cB *= gama * inv;
+      cU *= T;
+      gx += wq * cB * ex; gy += wq * cB * ey; gz += wq * cB * ez;
+      vx += wa * cB * ex; vy += wa * cB * ey; vz += wa * cB * ez;
+      ru += wq * cU;
+      rv += wa * cU;
+    }
+    // fixed-order wave sums
+    sx = wave_sum(sx); sy = wave_sum(sy); sz = wave_sum(sz);
+    gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz);
+    vx = wave_sum(vx); vy = wave_sum(vy); vz = wave_sum(vz);
+    ru = wave_sum(ru); rv = wave_sum(rv);
+    if (lane == 0) {
+      const float qa = d.q2ab_f[a], ba = d.abrw_f[a];
+      long long *p = d.acc_ch + 8 * (size_t)a;
+      fix_add(p + 0, -(double)(qa * sx + gx));
+      fix_add(p + 1, -(double)(qa * sy + gy));
+      fix_add(p + 2, -(double)(qa * sz + gz));
+      fix_add(p + 3, -(double)(ba * sx + vx));
+      fix_add(p + 4, -(double)(ba * sy + vy));
+      fix_add(p + 5, -(double)(ba * sz + vz));
+      if (aheavy) {                   // through the self volume: the tree derivative pass
+        fix_add(p + 6, (double)ru);
+        fix_add(p + 7, (double)rv);
+      }
     }
   }
-  if (!row) return;
-  const float qa = d.q2ab_f[a], ba = d.abrw_f[a];
-  long long *p = d.acc_ch + 8 * (size_t)a;
-  fix_add(p + 0, -(double)(qa * sx + gx));
-  fix_add(p + 1, -(double)(qa * sy + gy));
-  fix_add(p + 2, -(double)(qa * sz + gz));
-  fix_add(p + 3, -(double)(ba * sx + vx));
-  fix_add(p + 4, -(double)(ba * sy + vy));
-  fix_add(p + 5, -(double)(ba * sz + vz));
 }
 
 // ------------------------------------------------------------- the judge
@@ -602,7 +773,10 @@ size_t sg_mlp_lds() { return sizeof(MlpLds); }
 
 void launch_sg_front(const Dev &d, hipStream_t s) {
   if (!d.sg_on) return;
-  hipLaunchKernelGGL(k_sg_features, dim3(d.nsg_rows), dim3(kSgRows), 0, s, d);
+  hipLaunchKernelGGL(k_sg_guard, dim3(d.nsys), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_sg_cells, dim3(d.nsys), dim3(1024), 0, s, d);
+  hipLaunchKernelGGL(k_sg_nbl, dim3(d.nsg_rows), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_sg_features, dim3(d.nsg_rows), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_mode, dim3((d.nsys + 63) / 64), dim3(64), 0, s, d);
   const int ntiles = (d.n + kSgTile - 1) / kSgTile;
   hipLaunchKernelGGL(k_sg_mlp, dim3(std::min(ntiles, d.sg_grid)), dim3(256), sizeof(MlpLds), s, d);
@@ -667,7 +841,7 @@ void launch_sg_infer(const SgModel &m, const float *X, int n, float *s_out, floa
 
 void launch_sg_grad(const Dev &d, hipStream_t s) {
   if (!d.sg_on) return;
-  hipLaunchKernelGGL(k_sg_grad, dim3(d.nsg_rows), dim3(kSgRows), 0, s, d);
+  hipLaunchKernelGGL(k_sg_grad, dim3(d.nsg_rows), dim3(256), 0, s, d);
 }
 
 hipError_t prepare_surrogate() {

@@ -19,8 +19,9 @@ Gaussian coordinate perturbations (sigma 0.05 A, replay r seed 20263000 + r),
 against the same instances evaluated one agbnp3_ener_device call each.
 
 Learned surrogate (config 3), reported beside: a Born-radius MLP trained by
-the GPU kernels on ~2M atoms of config-2-style conformations (sizes seed
-20264999, conformation m seed 20264000 + m; radii from the plain kernels),
+the GPU kernels on ~2M atom rows of config-2-style conformations (sizes seed
+20264999, conformation m seed 20264000 + m, each as generated and displaced
+by sigma 0.05 A; radii from the plain kernels),
 then 50 held-out 6000-atom solutes (seeds 20265000 + k) evaluated as one
 batch with the surrogate (plain anchor every 10th call) replayed through a
 hipGraph, against the plain per-instance calls on the same perturbed frames
@@ -135,23 +136,28 @@ def train_surrogate(args, api, hip, synth, np):
     model.reserve(args.sg_train_atoms + 20000)
     sizes = synth.sizes(20264999, 8192, 300, 6000)
     total, m = 0, 0
+    # each conformation twice: as generated and with sigma 0.05 A displacements
+    # (the pairs train the network on differences between frames)
     while total < args.sg_train_atoms:
         group, gn = [], 0
-        while gn < 250000 and total + gn < args.sg_train_atoms:
+        while gn < 125000 and total + 2 * gn < args.sg_train_atoms:
             g = synth.generate(20264000 + m, int(sizes[m]))
             group.append(g)
             gn += g.natoms
             m += 1
         insts = [api.Instance(g) for g in group]
         b = api.Batch(insts)
-        xyz = [hip.DeviceArray.from_numpy(np.concatenate([getattr(g, k) for g in group])) for k in ("x", "y", "z")]
-        model.collect(b, xyz[0].addr, xyz[1].addr, xyz[2].addr)
-        for a in xyz:
-            a.free()
+        base = [np.concatenate([getattr(g, k) for g in group]) for k in ("x", "y", "z")]
+        rng = np.random.default_rng(20264500 + m)
+        for frame in (base, [a + rng.normal(0.0, 0.05, a.size) for a in base]):
+            xyz = [hip.DeviceArray.from_numpy(a) for a in frame]
+            model.collect(b, xyz[0].addr, xyz[1].addr, xyz[2].addr)
+            for a in xyz:
+                a.free()
         b.delete()
         for i in insts:
             i.delete()
-        total += gn
+        total += 2 * gn
     t_collect = time.perf_counter() - t0
     model.fit(margin=0.1, seed=20264002)
     hip.synchronize()
@@ -190,12 +196,14 @@ def bench_surrogate(args, model, api, hip, synth, np):
     for k in range(3):
         hip.copy_async(io.xyz[k].addr, frames[0][k].addr, 8 * n, hip.D2D, stream)
     b.tune(io.xyz[0].addr, io.xyz[1].addr, io.xyz[2].addr, reps=1)
+    b.set_nblist_reuse(1.0, every=10)
     b.sg_attach(model, 10)
     if b.ener_device(*io.ptrs()) != api.AGBNP_OK or b.sync() != api.AGBNP_OK:   # first anchor (call 1)
         raise RuntimeError("batched surrogate call failed")
     b.graph_capture(*io.ptrs())
     hip.synchronize()
     t0 = time.perf_counter()
+    sur_calls = 0
     for r in range(reps):
         for k in range(3):
             hip.copy_async(io.xyz[k].addr, frames[r + 1][k].addr, 8 * n, hip.D2D, stream)
@@ -208,6 +216,16 @@ def bench_surrogate(args, model, api, hip, synth, np):
     if rc != api.AGBNP_OK:
         raise RuntimeError("batched surrogate status error")
     err_est, modes = b.sg_stats()
+    # per-kernel times of surrogate calls (eager, after the timed replays)
+    b.kernel_times()
+    b.set_options(api.OPT_TIMING)
+    for r in range(2):
+        if b.ener_device(*io.ptrs()) != api.AGBNP_OK:
+            raise RuntimeError("batched surrogate call failed")
+    b.sync()
+    sur_kernels = b.kernel_times()
+    b.set_options(0)
+    _, modes_last = b.sg_stats()
     # plain form: one agbnp3_ener_device per instance, same frames
     pio = DeviceIO(hip, n, b.nsys)
     e_plain = [hip.DeviceArray(56 * b.nsys) for _ in range(reps)]
@@ -228,6 +246,30 @@ def bench_surrogate(args, model, api, hip, synth, np):
     t_plain = (time.perf_counter() - t0) / reps
     errs = [dg_rel_err(e_sur[r].to_numpy(np.float64, (b.nsys, 7)), e_plain[r].to_numpy(np.float64, (b.nsys, 7)), np)
             for r in range(reps)]
+    # plain batched form on the same frames (graph replays, no surrogate)
+    pb_insts = [api.Instance(g) for g in ss]
+    pb = api.Batch(pb_insts)
+    pbio = DeviceIO(hip, n, pb.nsys)
+    for k in range(3):
+        hip.copy_async(pbio.xyz[k].addr, frames[0][k].addr, 8 * n, hip.D2D, pb.stream())
+    pb.set_tuning(**b.get_tuning())
+    if pb.ener_device(*pbio.ptrs()) != api.AGBNP_OK or pb.sync() != api.AGBNP_OK:
+        raise RuntimeError("plain batch call failed")
+    pb.graph_capture(*pbio.ptrs())
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        for k in range(3):
+            hip.copy_async(pbio.xyz[k].addr, frames[r + 1][k].addr, 8 * n, hip.D2D, pb.stream())
+        if pb.graph_launch() != api.AGBNP_OK:
+            raise RuntimeError("graph launch failed")
+    pb.sync()
+    hip.synchronize()
+    t_pbatch = (time.perf_counter() - t0) / reps
+    pbio.free()
+    pb.delete()
+    for i in pb_insts:
+        i.delete()
     for a in [x for f in frames for x in f] + e_sur + e_plain:
         a.free()
     io.free()
@@ -236,10 +278,14 @@ def bench_surrogate(args, model, api, hip, synth, np):
     for i in insts:
         i.delete()
     return dict(workload=f"config 3: {len(ss)} held-out 6000-atom solutes (seeds 20265000+k) as one batch, "
-                         f"surrogate with a plain anchor every 10th call, hipGraph replays of sigma 0.05 A frames",
+                         f"surrogate with a plain anchor every 10th call, neighbour lists kept (1 A skin), "
+                         f"hipGraph replays of sigma 0.05 A frames",
                 natoms=int(n), calls=reps, surrogate_atom_evals_per_s=n / t_sur, surrogate_ms_per_call=1e3 * t_sur,
                 plain_atom_evals_per_s=n / t_plain, plain_ms_per_call=1e3 * t_plain, speedup=t_plain / t_sur,
+                plain_batched_ms_per_call=1e3 * t_pbatch, speedup_vs_plain_batched=t_pbatch / t_sur,
                 max_rel_dG_err=max(errs), rel_dG_err_per_call=errs,
+                systems_on_surrogate_last_call=int(np.sum(modes_last)), systems=int(b.nsys),
+                surrogate_kernel_ms=sur_kernels,
                 judge_abs_err_kcal=float(np.max(np.abs(err_est))))
 
 

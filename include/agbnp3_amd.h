@@ -128,13 +128,22 @@ int agbnp3_batch_sg_collect(int batch, int model, const double *x, const double 
 /* count, raw descriptors [count][66] and targets [count] (may be NULL) */
 int agbnp3_sg_dataset(int model, int *count, float *F, float *target);
 /* normalisation, training window (range widened by `margin` of its width
-   plus half an atom per shell, 0.02 A of radius, 0.1 e of charge), shuffled
-   bf16 training matrix */
+   plus half an atom per shell, 0.02 A of radius, 0.1 e of charge), and the
+   training rows: pairs of rows of one atom of one batch collected at two
+   coordinate sets (an atom collected once pairs with itself), in a shuffled
+   pair order, rows 2k and 2k+1 a pair; without any such pair, the rows in a
+   shuffled order */
 int agbnp3_sg_fit(int model, float margin, unsigned long long seed);
-/* Adam, mini-batches of B rows, lr decay^epoch; mean loss per epoch */
+/* loss per training row w_abs e^2 + w_diff c (e - e_pair)^2, e the error of
+   the normalised output and c = 1 / (mean squared normalised difference of
+   the pairs' targets); default w_abs = w_diff = 1 */
+int agbnp3_sg_set_loss(int model, float w_abs, float w_diff);
+/* training rows: count, dataset row of each (may be NULL), and c */
+int agbnp3_sg_train_rows(int model, int *count, int *rows, float *diff_scale);
+/* Adam, mini-batches of B rows (B even), lr decay^epoch; mean loss per epoch */
 int agbnp3_sg_train(int model, int epochs, int B, float lr, float decay, unsigned long long seed,
                     float *epoch_loss);
-/* gradients of the mean squared error over rows [r0, r0 + B) (no update) */
+/* gradients of the loss over training rows [r0, r0 + B), r0 even (no update) */
 int agbnp3_sg_grads(int model, int r0, int B, float *grad, float *loss);
 /* the model on raw descriptors F [n][66]: s [n], ds/dF [n][64] */
 int agbnp3_sg_infer(int model, const float *F, int n, float *s, float *dsdF);
@@ -144,6 +153,12 @@ int agbnp3_batch_sg_attach(int batch, int model, int every);
    (1 = surrogate) */
 int agbnp3_sg_stats(int tag, double *err, int *mode);
 int agbnp3_batch_sg_stats(int batch, double *err, int *mode);
+/* per system: builds of the descriptor neighbour lists (pairs within 13.5 A,
+   rebuilt after a move beyond 0.75 A) since a model was first attached, and
+   1 if an atom had more than 1536 neighbours (the system then stays on the
+   plain kernels) */
+int agbnp3_sg_lists(int tag, int *builds, int *overflow);
+int agbnp3_batch_sg_lists(int batch, int *builds, int *overflow);
 
 /* ---- host-only helpers (no GPU needed) ------------------------------------ */
 /* Q4 spline tables for radius types `radii` (R = r + 0.5): y, y2 sized

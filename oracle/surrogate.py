@@ -6,16 +6,19 @@ checker for the GPU kernels of agbnp3_amd/csrc/k_surrogate.hip:
 
 - local-environment descriptors: for atom i, four heavy-atom channels (by
   radius, R = r + 0.5) x 16 radial shells up to 12 A,
-      F[c, k] = sum_{j heavy, j != i, channel(j) = c} B3((d_ij - mu_k) / h) T(d_ij)
+      F[c, k] = sum_{j heavy, j != i, channel(j) = c} g_j B3((d_ij - mu_k) / h) T(d_ij)
   with h = 0.75 A, mu_k = h (k + 1/2), B3 the cubic B-spline (support
-  |t| < 2) and T a cosine taper from 1 at 10.5 A to 0 at 12 A; then R_i and
-  q_i (66 features);
+  |t| < 2), T a quintic smoothstep taper from 1 at 10.5 A to 0 at 12 A and g_j the
+  neighbour's self-volume fraction V'_j / V_j (the descreening's factor);
+  then R_i and q_i (66 features);
 - the MLP 66 -> 128 -> 128 -> 128 -> 1 with SiLU, evaluated as the GPU does:
   bf16 weights; normalised inputs and hidden activations carried as a pair
   of bf16 values (hi = bf16(v), lo = bf16(v - hi)); products accumulated
   exactly (here in FP64; FP32 on the GPU), FP32 biases;
 - its input gradient (bf16 deltas) and, for training, the weight gradients
-  of a mean squared error (bf16 activations and deltas in the products).
+  of a mean squared error on the values and on the differences between two
+  conformations of the same atom (bf16 activations and deltas in the
+  products).
 
 Everything is plain numpy in FP64 on bf16-rounded values.
 """
@@ -55,14 +58,14 @@ def b3p(t):
 
 
 def taper(d):
-    x = np.clip((d - TAPER0) / (RCUT - TAPER0), 0.0, 1.0)
-    return np.where(d >= RCUT, 0.0, 0.5 * (1.0 + np.cos(np.pi * x)))
+    """1 up to 10.5 A, 0 from 12 A, the quintic smoothstep between (C2)."""
+    x = np.clip((np.asarray(d, np.float64) - TAPER0) / (RCUT - TAPER0), 0.0, 1.0)
+    return 1.0 - x ** 3 * (10.0 - 15.0 * x + 6.0 * x * x)
 
 
 def taperp(d):
-    x = (d - TAPER0) / (RCUT - TAPER0)
-    inside = (d > TAPER0) & (d < RCUT)
-    return np.where(inside, -0.5 * np.pi / (RCUT - TAPER0) * np.sin(np.pi * x), 0.0)
+    x = np.clip((np.asarray(d, np.float64) - TAPER0) / (RCUT - TAPER0), 0.0, 1.0)
+    return -30.0 * x * x * (1.0 - x) ** 2 / (RCUT - TAPER0)
 
 
 def channel(R):
@@ -80,10 +83,12 @@ def shell_weights(d):
     return w, wp
 
 
-def features(pos, R, q, heavy):
-    """pos (n, 3), R (n,), q (n,), heavy (n,) bool -> F (n, 66), in FP64."""
+def features(pos, R, q, heavy, gamma=None):
+    """pos (n, 3), R (n,), q (n,), heavy (n,) bool, gamma (n,) neighbour
+    weights (None: 1) -> F (n, 66), in FP64."""
     pos = np.asarray(pos, np.float64)
     n = len(R)
+    gamma = np.ones(n) if gamma is None else np.asarray(gamma, np.float64)
     F = np.zeros((n, NFEAT))
     hv = np.flatnonzero(heavy)
     ch = channel(R[hv])
@@ -92,6 +97,7 @@ def features(pos, R, q, heavy):
         d = np.sqrt((dv * dv).sum(1))
         m = (hv != i) & (d < RCUT)
         w, _ = shell_weights(d[m])
+        w = w * gamma[hv[m]][:, None]
         for c in range(NCHAN):
             F[i, c * NSHELL:(c + 1) * NSHELL] = w[ch[m] == c].sum(0)
     F[:, 64] = R
@@ -99,24 +105,28 @@ def features(pos, R, q, heavy):
     return F
 
 
-def feature_gradient(pos, R, heavy, G):
-    """Given G (n, 64) = dE/dF for the shell features, return dE/dpos (n, 3)."""
+def feature_gradient(pos, R, heavy, G, gamma=None):
+    """Given G (n, 64) = dE/dF for the shell features, return dE/dpos (n, 3)
+    at fixed weights and dE/dgamma (n,)."""
     pos = np.asarray(pos, np.float64)
     n = len(R)
+    gamma = np.ones(n) if gamma is None else np.asarray(gamma, np.float64)
     hv = np.flatnonzero(heavy)
     ch = channel(R[hv])
     g = np.zeros((n, 3))
+    dgam = np.zeros(n)
     for i in range(n):
         dv = pos[hv] - pos[i]
         d = np.sqrt((dv * dv).sum(1))
         m = (hv != i) & (d < RCUT)
-        _, wp = shell_weights(d[m])
-        gi = np.take_along_axis(G[i].reshape(NCHAN, NSHELL)[ch[m]], np.arange(NSHELL)[None, :], 1)
-        coef = (gi * wp).sum(1) / d[m]              # dE/dd / d
+        w, wp = shell_weights(d[m])
+        gi = G[i].reshape(NCHAN, NSHELL)[ch[m]]     # dE/dF of i in each neighbour's channel
+        coef = (gi * wp).sum(1) * gamma[hv[m]] / d[m]   # dE/dd / d
         contrib = coef[:, None] * dv[m]             # along x_j - x_i
         g[i] -= contrib.sum(0)
         np.add.at(g, hv[m], contrib)
-    return g
+        np.add.at(dgam, hv[m], (gi * w).sum(1))
+    return g, dgam
 
 
 class Params:
@@ -191,16 +201,23 @@ def input_gradient(P, cache, rnd=bf16):
     return dX * P.ysd / P.xsd
 
 
-def loss_and_weight_gradients(P, F, target, rnd=bf16, act=split):
-    """Mean squared error of the normalised output and its parameter gradients;
-    the weight-gradient products take the activations rounded by rnd."""
+def loss_and_weight_gradients(P, F, target, rnd=bf16, act=split, w_abs=1.0, w_diff=0.0):
+    """Loss of the normalised output and its parameter gradients; the
+    weight-gradient products take the activations rounded by rnd.
+
+    Rows 2k and 2k+1 form a pair (the same atom in two conformations); with
+    e = y - t the loss is mean over rows of w_abs e_r^2 + w_diff (e_r - e_p)^2,
+    p the row's partner (none for a last, odd row)."""
     s, (X, z1, a1, z2, a2, z3, a3, y) = forward(P, F, rnd, act)
     X, a1, a2 = rnd(X), rnd(a1), rnd(a2)
     t = (target - P.ymu) / P.ysd
     n = len(t)
     e = y - t
-    loss = float(np.mean(e * e))
-    dy = 2.0 * e / n
+    partner = np.arange(n) ^ 1
+    partner[partner >= n] = np.arange(n)[partner >= n]
+    de = e - e[partner]
+    loss = float(np.mean(w_abs * e * e + w_diff * de * de))
+    dy = (2.0 * w_abs * e + 4.0 * w_diff * de) / n
     g = {"w4": a3.T @ dy, "b4": dy.sum()}
     d3 = rnd(dy[:, None] * P.w4[None, :] * silup(z3))
     g["W3"], g["b3"] = a2.T @ d3, d3.sum(0)

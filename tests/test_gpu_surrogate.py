@@ -1,7 +1,8 @@
 """The learned Born-radius surrogate on the GPU (agbnp3_sg_*) against its CPU
 reference (oracle/surrogate.py) and against the plain kernels.
 
-- descriptors of the GPU feature kernel vs the FP64 brute-force reference;
+- descriptors of the GPU feature kernel vs the FP64 brute-force reference
+  (neighbours weighted by the oracle's self-volume fractions);
 - training targets vs the FP64 oracle's descreening sums;
 - MLP inference (bf16 MFMA) and its input gradient vs the bf16-mirroring
   CPU reference; training gradients of the whole set vs the reference;
@@ -82,7 +83,7 @@ def test_descriptors_and_targets_match_the_references(small_set):
         R = (s.r + 0.5).astype(np.float32).astype(np.float64)
         pos = np.stack([s.x, s.y, s.z], 1).astype(np.float32).astype(np.float64)
         heavy = ~np.isin(np.arange(s.natoms), s.ihydrogen)
-        Fr = ref.features(pos[i2e], R[i2e], s.charge.astype(np.float32)[i2e], heavy[i2e])
+        Fr = ref.features(pos[i2e], R[i2e], s.charge.astype(np.float32)[i2e], heavy[i2e], orc.internal("sp"))
         Fg = F[o:o + s.natoms].astype(np.float64)
         assert np.allclose(Fg[:, :64], Fr[:, :64], rtol=1e-5, atol=1e-4)
         assert np.array_equal(Fg[:, 64], R[i2e].astype(np.float32))
@@ -90,6 +91,53 @@ def test_descriptors_and_targets_match_the_references(small_set):
         s_ref = 4.0 * np.pi * (1.0 / (R[i2e] - 0.5) - beta)
         assert np.allclose(t[o:o + s.natoms], s_ref, rtol=1e-5, atol=1e-5)
         o += s.natoms
+
+
+def test_descriptor_lists_follow_the_coordinates(small_set):
+    """Descriptors over a sequence of frames: small moves keep the neighbour
+    lists, a move beyond half their 1.5 A skin rebuilds them, and every
+    frame's descriptors equal the brute-force reference at its coordinates."""
+    s = small_set[0][2]
+    m = api.Surrogate(seed=2)
+    inst = api.Instance(s)
+    b = api.Batch([inst])
+    try:
+        m.reserve(4 * s.natoms)
+        rng = np.random.default_rng(20263300)
+        x, y, z = s.x.copy(), s.y.copy(), s.z.copy()
+        frames = [(x, y, z)]
+        x, y, z = (a + rng.normal(0.0, 0.05, a.size) for a in (x, y, z))
+        frames.append((x, y, z))
+        x = x.copy()
+        x[5] += 1.0                                   # beyond 0.75 A: rebuild
+        frames.append((x, y, z))
+        x, y, z = (a + rng.normal(0.0, 0.05, a.size) for a in (x, y, z))
+        frames.append((x, y, z))
+        builds = []
+        for f in frames:
+            d = dev_coords(*f)
+            m.collect(b, d[0].addr, d[1].addr, d[2].addr)
+            for a in d:
+                a.free()
+            builds.append(int(b.sg_lists()[0][0]))
+        assert builds == [1, 1, 2, 2]
+        assert b.sg_lists()[1][0] == 0
+        F, _ = m.dataset()
+        for k, f in enumerate(frames):
+            sk = s.with_coords(*f)
+            orc = oracle.Oracle(sk)
+            orc.ener()
+            i2e = orc.int2ext()
+            R = (s.r + 0.5).astype(np.float32).astype(np.float64)
+            pos = np.stack(f, 1).astype(np.float32).astype(np.float64)
+            heavy = ~np.isin(np.arange(s.natoms), s.ihydrogen)
+            Fr = ref.features(pos[i2e], R[i2e], s.charge.astype(np.float32)[i2e], heavy[i2e], orc.internal("sp"))
+            Fg = F[k * s.natoms:(k + 1) * s.natoms].astype(np.float64)
+            assert np.allclose(Fg[:, :64], Fr[:, :64], rtol=1e-5, atol=1e-4), k
+    finally:
+        b.delete()
+        inst.delete()
+        m.delete()
 
 
 def test_mlp_inference_matches_cpu_reference(small_set):
@@ -105,13 +153,14 @@ def test_mlp_inference_matches_cpu_reference(small_set):
     assert np.sqrt(np.mean((d_gpu - d_ref) ** 2)) <= 1e-2 * np.sqrt(np.mean(d_ref ** 2))
 
 
-def test_training_gradients_match_cpu_reference(small_set):
-    ss, m = small_set
+def compare_training_gradients(m, w_abs, w_diff):
     F, t = m.dataset()
+    rows, c = m.train_rows()
     P = cpu_params(m)
-    n = len(t)
-    loss_g, g = m.grads(0, n)                         # all rows: order-independent sums
-    loss_r, gr = ref.loss_and_weight_gradients(P, F.astype(np.float64), t.astype(np.float64))
+    n = len(rows)
+    loss_g, g = m.grads(0, n)                         # every training row at once
+    loss_r, gr = ref.loss_and_weight_gradients(P, F[rows].astype(np.float64), t[rows].astype(np.float64),
+                                               w_abs=w_abs, w_diff=w_diff * c)
     assert abs(loss_g - loss_r) <= 1e-3 * loss_r
     o = 0
     for name, shape in api.Surrogate.LAYOUT:
@@ -119,6 +168,33 @@ def test_training_gradients_match_cpu_reference(small_set):
         a, b = g[o:o + size], np.ravel(gr[name])
         assert np.sqrt(np.mean((a - b) ** 2)) <= 2e-2 * np.sqrt(np.mean(b ** 2)) + 1e-9, name
         o += size
+    return rows, c
+
+
+def test_training_gradients_match_cpu_reference(small_set):
+    ss, m = small_set
+    rows, c = compare_training_gradients(m, 1.0, 1.0)
+    assert c == 0.0 and np.array_equal(np.sort(rows), np.arange(len(rows)))   # one frame: no pairs
+
+
+def test_pair_loss_gradients_match_cpu_reference(small_set):
+    """Two frames of each atom: training rows come in pairs of one atom, the
+    loss adds the squared error of the pair's difference."""
+    ss, _ = small_set
+    m = api.Surrogate(seed=4)
+    try:
+        m.reserve(4096)
+        collect(m, ss[:2], frames=2, sigma=0.05, seed=20263400)
+        m.fit(margin=0.25, seed=6)
+        m.set_loss(0.5, 2.0)
+        rows, c = compare_training_gradients(m, 0.5, 2.0)
+        n_atoms = ss[0].natoms + ss[1].natoms
+        assert len(rows) == 2 * n_atoms and c > 0.0
+        # each pair: one atom (same row modulo the frame), two frames
+        assert np.all(rows[0::2] % n_atoms == rows[1::2] % n_atoms)
+        assert np.all(rows[0::2] != rows[1::2])
+    finally:
+        m.delete()
 
 
 def test_anchor_calls_equal_the_plain_kernels(small_set):
@@ -144,6 +220,13 @@ def test_anchor_calls_equal_the_plain_kernels(small_set):
 
 
 def test_surrogate_gradients_are_derivatives_of_its_energy(small_set):
+    """On surrogate calls, the gradients of E_GB and E_vdW are the
+    derivatives of their energies: directional derivatives along random
+    displacements of every coordinate (normal, scaled by h = 0.005 A: the
+    energy's curvature along such collective moves makes larger steps
+    inexact) against central differences, and single-coordinate ones at a
+    looser bound (the MLP's activations carry 16 significant bits, which
+    limits a one-atom difference quotient)."""
     ss, m = small_set
     s = ss[0]
     inst = api.Instance(s)
@@ -153,7 +236,18 @@ def test_surrogate_gradients_are_derivatives_of_its_energy(small_set):
         out = inst.ener()
         assert inst.sg_stats()[1] == 1
         rng = np.random.default_rng(9)
-        h = 1e-2
+        base = np.stack([s.x, s.y, s.z], 1)
+        for trial in range(3):
+            v = rng.normal(size=base.shape)           # every coordinate moves by ~h
+            h = 5e-3
+            ep = inst.ener(*(base + h * v).T.copy())
+            em = inst.ener(*(base - h * v).T.copy())
+            assert inst.sg_stats()[1] == 1
+            for e, g in (("egb", "dgbdr"), ("evdw", "dvwdr")):
+                fd = (ep[e] - em[e]) / (2 * h)
+                an = float(np.sum(out[g] * v))
+                assert abs(fd - an) <= 1e-2 * abs(an) + 2e-3 * np.linalg.norm(out[g]) + 3e-4, (trial, e, fd, an)
+        h = 2e-2
         for a in rng.choice(s.natoms, 4, replace=False):
             for c in range(3):
                 xyz = [s.x.copy(), s.y.copy(), s.z.copy()]
@@ -164,7 +258,7 @@ def test_surrogate_gradients_are_derivatives_of_its_energy(small_set):
                 for e, g in (("egb", "dgbdr"), ("evdw", "dvwdr")):
                     fd = (ep[e] - em[e]) / (2 * h)
                     an = out[g][a, c]
-                    assert abs(fd - an) <= 2e-2 * max(1.0, abs(an)), (a, c, e, fd, an)
+                    assert abs(fd - an) <= 5e-2 * max(1.0, abs(an)), (a, c, e, fd, an)
     finally:
         inst.delete()
 
