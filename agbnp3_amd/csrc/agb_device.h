@@ -49,6 +49,13 @@ __device__ __forceinline__ int next_item(unsigned *head, int *slot) {
   __syncthreads();
   return *slot;
 }
+// the same, taking `n` consecutive items per grab
+__device__ __forceinline__ int next_item_n(unsigned *head, int *slot, int n) {
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(head, (unsigned)n);
+  __syncthreads();
+  return *slot;
+}
 
 // ------------------------------------------------------------------ cells
 __device__ __forceinline__ void cell_coords(const GridParams &g, float x, float y, float z, int &cx, int &cy,
@@ -182,6 +189,39 @@ __device__ __forceinline__ void bin_atoms(const GridParams &g, const float4 *pos
     }
   }
   __syncthreads();
+}
+
+// The 27-cell stencil around cell (cx, cy, cz) as 9 segments of the
+// cell-sorted atoms (the three x-neighbour cells of one (y, z) are
+// consecutive in the sort), their bounds loaded by lanes 0..8 at once.
+// Candidate k (0 <= k < the returned total) of the stencil is atom slot
+// stencil_slot(start, off, k); candidates come segment by segment.
+__device__ __forceinline__ int stencil_segments(const GridParams &g, const int *cstart, int cx, int cy, int cz,
+                                                int (&start)[9], int (&off)[9]) {
+  const int lane = threadIdx.x & 63;
+  int sbeg = 0, slen = 0;
+  if (lane < 9) {
+    const int zz = cz + lane / 3 - 1, yy = cy + lane % 3 - 1;
+    if (zz >= 0 && zz < g.dim[2] && yy >= 0 && yy < g.dim[1]) {
+      const int c0 = (zz * g.dim[1] + yy) * g.dim[0];
+      sbeg = cstart[c0 + max(cx - 1, 0)];
+      slen = cstart[c0 + min(cx + 1, g.dim[0] - 1) + 1] - sbeg;
+    }
+  }
+  const int incl = wave_incl_scan(slen);
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    start[q] = __shfl(incl - slen, q, 64);
+    off[q] = __shfl(sbeg, q, 64) - start[q];
+  }
+  return __shfl(incl, 8, 64);
+}
+__device__ __forceinline__ int stencil_slot(const int (&start)[9], const int (&off)[9], int k) {
+  int o = off[0];
+#pragma unroll
+  for (int q = 1; q < 9; q++)
+    if (k >= start[q]) o = off[q];      // empty segments share the next one's start
+  return o + k;
 }
 
 // Block index remapped so that each XCD (blocks are dealt to the 8 XCDs in

@@ -300,39 +300,28 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
   if (active) {
     int cx, cy, cz;
     cell_coords(g, W.pos[0], W.pos[1], W.pos[2], cx, cy, cz);
-    for (int dz = -1; dz <= 1; dz++) {
-      const int zz = cz + dz;
-      if (zz < 0 || zz >= g.dim[2]) continue;
-      for (int dy = -1; dy <= 1; dy++) {
-        const int yy = cy + dy;
-        if (yy < 0 || yy >= g.dim[1]) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          const int xx = cx + dx;
-          if (xx < 0 || xx >= g.dim[0]) continue;
-          const int c = (zz * g.dim[1] + yy) * g.dim[0] + xx;
-          const int beg = cstart[c], end = cstart[c + 1];
-          for (int base = beg; base < end; base += 64) {
-            const int k = base + lane;
-            bool pass = false;
-            int j = -1;
-            if (k < end) {
-              j = d.cell_atoms[k];
-              const float4 pj = d.pos[j];
-              const float ex = __fsub_rn(pj.x, W.pos[0]), ey = __fsub_rn(pj.y, W.pos[1]),
-                          ez = __fsub_rn(pj.z, W.pos[2]);
-              const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ex, ex), __fmul_rn(ey, ey)), __fmul_rn(ez, ez));
-              const float u = __fmul_rn(__fadd_rn(pj.w, kHbRadius), kNbOffset);
-              pass = d2 < __fmul_rn(u, u);
-            }
-            const unsigned long long m = __ballot(pass);
-            if (pass) {
-              const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
-              if (slot < kWsCap) cj[w][slot] = j;
-            }
-            cnt += __popcll(m);
-          }
-        }
+    // the stencil's candidates 64 at a time; sorted by atom index below
+    int start[9], off[9];
+    const int total = stencil_segments(g, cstart, cx, cy, cz, start, off);
+    for (int base = 0; base < total; base += 64) {
+      const int k = base + lane;
+      bool pass = false;
+      int j = -1;
+      if (k < total) {
+        j = d.cell_atoms[stencil_slot(start, off, k)];
+        const float4 pj = d.pos[j];
+        const float ex = __fsub_rn(pj.x, W.pos[0]), ey = __fsub_rn(pj.y, W.pos[1]),
+                    ez = __fsub_rn(pj.z, W.pos[2]);
+        const float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ex, ex), __fmul_rn(ey, ey)), __fmul_rn(ez, ez));
+        const float u = __fmul_rn(__fadd_rn(pj.w, kHbRadius), kNbOffset);
+        pass = d2 < __fmul_rn(u, u);
       }
+      const unsigned long long m = __ballot(pass);
+      if (pass) {
+        const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
+        if (slot < kWsCap) cj[w][slot] = j;
+      }
+      cnt += __popcll(m);
     }
   }
   if (cnt > kWsCap) {

@@ -139,37 +139,27 @@ __global__ __launch_bounds__(64 * kNbWaves) void k_nblist(Dev d) {
     int ccnt = 0;
     int cx, cy, cz;
     cell_coords(g, pi.x, pi.y, pi.z, cx, cy, cz);
-    for (int dz = -1; dz <= 1; dz++) {
-      const int zz = cz + dz;
-      if (zz < 0 || zz >= g.dim[2]) continue;
-      for (int dy = -1; dy <= 1; dy++) {
-        const int yy = cy + dy;
-        if (yy < 0 || yy >= g.dim[1]) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          const int xx = cx + dx;
-          if (xx < 0 || xx >= g.dim[0]) continue;
-          const int c = (zz * g.dim[1] + yy) * g.dim[0] + xx;
-          const int beg = cstart[c], end = cstart[c + 1];
-          for (int base = beg; base < end; base += 64) {
-            const int k = base + lane;
-            bool pass = false, cpass = false;
-            int j = -1;
-            float d2 = 0.f;
-            if (k < end) {
-              j = d.cell_atoms[k];
-              if (j > i) test(j, reuse, pass, cpass, d2);
-            }
-            keep(pass, j, d2);
-            if (reuse) {
-              const unsigned long long mc = __ballot(cpass);
-              if (cpass) {
-                const int slot = ccnt + __popcll(mc & ((1ull << lane) - 1ull));
-                if (slot < kCandCap) cand[slot] = j;
-              }
-              ccnt += __popcll(mc);
-            }
-          }
+    // candidates gathered 64 at a time over the whole stencil; the (d2, j)
+    // sort below makes the list independent of the order of enumeration
+    int start[9], off[9];
+    const int total = stencil_segments(g, cstart, cx, cy, cz, start, off);
+    for (int base = 0; base < total; base += 64) {
+      const int k = base + lane;
+      bool pass = false, cpass = false;
+      int j = -1;
+      float d2 = 0.f;
+      if (k < total) {
+        j = d.cell_atoms[stencil_slot(start, off, k)];
+        if (j > i) test(j, reuse, pass, cpass, d2);
+      }
+      keep(pass, j, d2);
+      if (reuse) {
+        const unsigned long long mc = __ballot(cpass);
+        if (cpass) {
+          const int slot = ccnt + __popcll(mc & ((1ull << lane) - 1ull));
+          if (slot < kCandCap) cand[slot] = j;
         }
+        ccnt += __popcll(mc);
       }
     }
     if (reuse && lane == 0) {
@@ -213,10 +203,31 @@ __global__ __launch_bounds__(64 * kNbWaves) void k_nblist(Dev d) {
     }
     d.nbr[(size_t)h * d.cap + rank] = je;
   }
-  if (lane == 0) {
-    d.nne[h] = m;
-    atomicAdd(&d.counters[2], (unsigned long long)m);
-    atomicMax(&d.counters[3], (unsigned long long)m);
+  if (lane == 0) d.nne[h] = m;
+}
+
+// near-pair total and longest row (counters 2, 3): one add and one max per
+// block rather than per row (same-address atomics serialise in L2)
+__global__ __launch_bounds__(256) void k_nbl_count(Dev d) {
+  __shared__ unsigned long long ssum[4];
+  __shared__ unsigned smax[4];
+  const int h = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned v = h < d.nh ? (unsigned)d.nne[h] : 0u;
+  unsigned long long sum = v;
+  unsigned mx = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o, 64);
+    mx = max(mx, (unsigned)__shfl_xor((int)mx, o, 64));
+  }
+  if (lane == 0) { ssum[w] = sum; smax[w] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = ssum[0] + ssum[1] + ssum[2] + ssum[3];
+    const unsigned m = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
+    if (t) atomicAdd(&d.counters[2], t);
+    if (m) atomicMax(&d.counters[3], (unsigned long long)m);
   }
 }
 
@@ -242,6 +253,7 @@ void launch_nblist(const Dev &d, hipStream_t s) {
     hipLaunchKernelGGL(k_nblist<128>, dim3(nblk), dim3(64 * kNbWaves), 0, s, d);
   else
     hipLaunchKernelGGL(k_nblist<256>, dim3(nblk), dim3(64 * kNbWaves), 0, s, d);
+  hipLaunchKernelGGL(k_nbl_count, dim3((d.nh + 255) / 256), dim3(256), 0, s, d);
 }
 
 }  // namespace agb

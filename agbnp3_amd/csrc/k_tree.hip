@@ -300,7 +300,8 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
 // depth-first walk of the roots the breadth-first kernel queued (overflow);
 // the grid covers every heavy row, blocks past the queue length exit
 template <int CAP, bool PB>
-__device__ void tree_dfs_root(const Dev &d, TreeLds<CAP, PB> &S, int h) {
+__device__ void tree_dfs_root(const Dev &d, TreeLds<CAP, PB> &S, int h, unsigned long long &tk,
+                              unsigned long long &te) {
   const int lane = threadIdx.x;
   const int i = d.hrow[h];
   const int m = d.nne[h];
@@ -363,10 +364,8 @@ __device__ void tree_dfs_root(const Dev &d, TreeLds<CAP, PB> &S, int h) {
       for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], S.acc[v][s]);
     }
   }
-  if (!PB && lane == 0) {
-    atomicAdd(&d.counters[0], (unsigned long long)nkept);
-    atomicAdd(&d.counters[1], (unsigned long long)neval);
-  }
+  tk += nkept;                          // wave-uniform; added to the counters once per block
+  te += neval;
   __syncthreads();
 }
 
@@ -377,12 +376,17 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
   __shared__ int slot;
   const int count = (int)d.counters[6 + (PB ? 1 : 0)];
   unsigned *head = &d.queue[PB ? Q_TREE_FB : Q_TREE_FA];
+  unsigned long long tk = 0, te = 0;
   for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
-    tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(size_t)(2 + (PB ? 1 : 0)) * d.nh + k]);
+    tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(size_t)(2 + (PB ? 1 : 0)) * d.nh + k], tk, te);
+  if (!PB && threadIdx.x == 0 && (tk | te)) {
+    atomicAdd(&d.counters[0], tk);
+    atomicAdd(&d.counters[1], te);
+  }
 }
 
 // ------------------------------------------------------ breadth-first tree
-constexpr int kLevelCap = 256;          // nodes of one order held in LDS
+constexpr int kLevelCap = 128;          // nodes of one order held in LDS
 constexpr int kLevelCapBig = 1536;      // second tier (dynamic LDS) for the largest trees
 
 template <int CAP, bool PB, int LC>
@@ -491,7 +495,8 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, 
 // one root of the breadth-first tree (the whole wave); results scattered to
 // the atoms, or the root queued for the depth-first kernel on overflow
 template <int CAP, bool PB, int LC>
-__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int list) {
+__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int list, unsigned long long &tk,
+                              unsigned long long &te) {
   const int lane = threadIdx.x;
   const int i = d.hrow[h];
   const int m = d.nne[h];
@@ -582,9 +587,16 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int l
       for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], S.acc[v][s]);
     }
   }
-  if (!PB && lane == 0) {
-    atomicAdd(&d.counters[0], (unsigned long long)nkept);
-    atomicAdd(&d.counters[1], (unsigned long long)neval);
+  tk += nkept;                          // wave-uniform; added to the counters once per block
+  te += neval;
+}
+
+// overlaps kept and evaluated by one persistent block (pass A only)
+__device__ __forceinline__ void add_tree_counts(const Dev &d, bool pass_b, unsigned long long tk,
+                                                unsigned long long te) {
+  if (!pass_b && threadIdx.x == 0 && (tk | te)) {
+    atomicAdd(&d.counters[0], tk);
+    atomicAdd(&d.counters[1], te);
   }
 }
 
@@ -596,8 +608,13 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   __shared__ BfsLds<CAP, PB, kLevelCap> S;
   __shared__ int slot;
   unsigned *head = &d.queue[PB ? Q_TREE_B : Q_TREE_A];
-  for (int h = next_item(head, &slot); h < d.nh; h = next_item(head, &slot))
-    tree_bfs_root<CAP, PB, kLevelCap>(d, S, h, 0);
+  // roots in runs of `run` per queue grab when there are many per block
+  // (one returning atomic on the queue head per grab)
+  const int run = max(1, min(8, d.nh / (int)(gridDim.x * 8)));
+  unsigned long long tk = 0, te = 0;
+  for (int h0 = next_item_n(head, &slot, run); h0 < d.nh; h0 = next_item_n(head, &slot, run))
+    for (int h = h0; h < min(h0 + run, d.nh); h++) tree_bfs_root<CAP, PB, kLevelCap>(d, S, h, 0, tk, te);
+  add_tree_counts(d, PB, tk, te);
 }
 
 // second tier: the roots the first tier queued, with 1536-node levels in
@@ -609,8 +626,10 @@ __global__ __launch_bounds__(64) void k_tree_bfs_big(Dev d) {
   __shared__ int slot;
   const int count = (int)d.counters[4 + (PB ? 1 : 0)];
   unsigned *head = &d.queue[PB ? Q_TREE_BB : Q_TREE_BA];
+  unsigned long long tk = 0, te = 0;
   for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
-    tree_bfs_root<CAP, PB, kLevelCapBig>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1);
+    tree_bfs_root<CAP, PB, kLevelCapBig>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1, tk, te);
+  add_tree_counts(d, PB, tk, te);
 }
 
 template <int CAP, bool PB>
