@@ -266,7 +266,7 @@ __device__ __forceinline__ void q4_lookup(const float2 *__restrict__ tab, float 
   dq = 0.f;
   if (k >= 0 && k <= kTblN - 2) {
     const float2 n0 = tab[k], n1 = tab[k + 1];
-    const float A = (float)(k + 1) - xh, B = xh - (float)k;
+    const float B = xh - (float)k, A = 1.f - B;   // 1 - B rounds to the same value as (k + 1) - xh
     const float A2 = A * A, B2 = B * B;
     if (WANT_Q) q = A * n0.x + B * n1.x + ((A2 * A - A) * n0.y + (B2 * B - B) * n1.y) * dp2;
     if (WANT_DQ) dq = (n1.x - n0.x) * dxinv - ((3.f * A2 - 1.f) * n0.y + sgn2 * (3.f * B2 - 1.f) * n1.y) * dp1;

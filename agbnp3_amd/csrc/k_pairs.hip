@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
   __shared__ float2 tab_rc[kTblN], tab_cr[kTblN];   // row <- column, column <- row
   __shared__ float4 cpos[256];                      // x, y, z, 1/R
   __shared__ float4 cpar[256];                      // sp (0: H), q2ab, abrw, heavy
-  __shared__ float colacc[4][8][256];               // [wave][gb xyz, vw xyz, U, V][column]
+  __shared__ float colacc[8][256];                  // [gb xyz, vw xyz, U, V][column]: waves add in turn
   __shared__ int slot;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
@@ -265,6 +265,8 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
     const float iRk = 1.0f / pk.w;
     __syncthreads();
     float rg[3] = {0.f, 0.f, 0.f}, rv[3] = {0.f, 0.f, 0.f}, rU = 0.f, rV = 0.f;
+    float cacc[4][8];                                // this wave's column sums, per 64-column chunk
+#pragma unroll
     for (int c = 0; c < 4; c++) {
       float ca[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       for (int s = 0; s < 64; s++) {
@@ -275,7 +277,8 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
         const bool pair = (kr >= 0) && (kheavy || jheavy) && !(diag && col == w * 64 + lane);
         if (pair) {
           const float ex = pj.x - pk.x, ey = pj.y - pk.y, ez = pj.z - pk.z;
-          const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+          const float d2 = ex * ex + ey * ey + ez * ez;
+          const float id = rsqrtf(d2), dist = d2 * id;
           float hgb = 0.f, hvw = 0.f;
           if (jheavy) {                                      // k descreened by j
             float qv, dq;
@@ -297,7 +300,6 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
             rU += pp.y * qv;
             rV += pp.z * qv;
           }
-          const float id = 1.0f / dist;
           hgb *= id;
           hvw *= id;
           rg[0] += hgb * ex; rg[1] += hgb * ey; rg[2] += hgb * ez;
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
         for (int q = 0; q < 8; q++) ca[q] = wave_rot(ca[q]);
       }
 #pragma unroll
-      for (int q = 0; q < 8; q++) colacc[w][q][c * 64 + lane] = ca[q];
+      for (int q = 0; q < 8; q++) cacc[c][q] = ca[q];
     }
     if (kr >= 0) {
       long long *pr = d.acc_ch + 8 * (size_t)kr;
@@ -320,14 +322,23 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
       fix_add(pr + 6, (double)rU);
       fix_add(pr + 7, (double)rV);
     }
-    __syncthreads();
+    // column sums over the four waves in a fixed order (wave 0 + 1 + 2 + 3)
+    for (int ww = 0; ww < 4; ww++) {
+      if (w == ww) {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+          for (int q = 0; q < 8; q++)
+            colacc[q][c * 64 + lane] = ww == 0 ? cacc[c][q] : colacc[q][c * 64 + lane] + cacc[c][q];
+      }
+      __syncthreads();
+    }
     const int j = d.cperm[item.y * 256 + threadIdx.x];
     if (!diag && j >= 0) {
       long long *pc = d.acc_ch + 8 * (size_t)j;
       const int t = threadIdx.x;
 #pragma unroll
-      for (int q = 0; q < 8; q++)
-        fix_add(pc + q, (double)colacc[0][q][t] + colacc[1][q][t] + colacc[2][q][t] + colacc[3][q][t]);
+      for (int q = 0; q < 8; q++) fix_add(pc + q, (double)colacc[q][t]);
     }
   }
 }
