@@ -74,7 +74,8 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
       const int nc = min(kTile, c1 - base);
       float tacc = 0.f;
       if (i >= 0) {
-        for (int c = 0; c < nc; c++) {
+#pragma unroll 4
+        for (int c = 0; c < nc; c++) {                // independent columns overlap; sums stay in order
           const float4 pj = cpos[c];
           const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
           const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
