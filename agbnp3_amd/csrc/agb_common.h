@@ -230,15 +230,23 @@ struct Dev {
   // block b: neighbour k at sg_nbr[(b * kSgBlk + r) * kSgNbrCap + k],
   // heavy neighbours from k = 0, hydrogens (heavy rows only) down from
   // k = kSgNbrCap - 1; sg_nn[a] = (heavy, hydrogen) counts.
-  int *sg_nbr;
+  int *sg_nbr;                         // neighbour slots (positions in sg_catoms)
   int2 *sg_nn;
   float4 *sg_ref;                      // [n] positions the lists were built from
+  int *sg_slot;                        // [n] slot of each atom in sg_catoms
+  // per call, in slot order: position and R, self-volume fraction (0 for
+  // hydrogens), and (q2ab, abrw) for the gradient pass; sg_F, sg_X, sg_s and
+  // sg_dsdF are also in slot order
+  float4 *sg_spos;
+  float *sg_sgam;
+  float2 *sg_sw;
   int *sg_nbl;                         // per system [0] built, [1] rebuild this call, [2] builds, [3] overflow
   GridParams *sg_grid_p;               // per system all-atom cell grid
   int *sg_cstart, *sg_catoms, *sg_ctmp;  // [nsys][kMaxCells + 1], [n], [n]
-  float *sg_F;                         // raw shell descriptors [n][64]
-  float *sg_X;                         // normalised input [n][kSgKPad]
-  float *sg_s, *sg_dsdF;               // predicted s [n], ds/dF [n][64]
+  float *sg_F;                         // raw shell descriptors [n][64] (slot order)
+  float *sg_X;                         // normalised input [n][kSgKPad] (slot order)
+  float *sg_s;                         // predicted s [n] (slot order)
+  float *sg_dsdF;                      // ds/dF by channel: [4][n][16] (slot order)
   float *sg_target;                    // plain descreening sum of the call [n]
   double *sg_c, *sg_delta;             // anchor corrections, surrogate error at the anchor [n]
   double *sg_err;                      // per system: first-order Delta G error estimate

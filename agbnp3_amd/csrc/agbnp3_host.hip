@@ -1767,7 +1767,8 @@ static int pack_sg_attach(Pack &P, int model, int every) {
         dalloc(L, &d.sg_c, nn) || dalloc(L, &d.sg_delta, nn) || dalloc(L, &d.sg_err, ns) ||
         dalloc(L, &d.sg_nbr, rows.size() * kSgNbrCap * kSgBlk) || dalloc(L, &d.sg_nn, nn) ||
         dalloc(L, &d.sg_ref, nn) || dalloc(L, &d.sg_nbl, 4 * ns) || dalloc(L, &d.sg_grid_p, ns) ||
-        dalloc(L, &d.sg_cstart, ns * (kMaxCells + 1)) || dalloc(L, &d.sg_catoms, nn) || dalloc(L, &d.sg_ctmp, nn))
+        dalloc(L, &d.sg_cstart, ns * (kMaxCells + 1)) || dalloc(L, &d.sg_catoms, nn) || dalloc(L, &d.sg_ctmp, nn) ||
+        dalloc(L, &d.sg_slot, nn) || dalloc(L, &d.sg_spos, nn) || dalloc(L, &d.sg_sgam, nn) || dalloc(L, &d.sg_sw, nn))
       return AGBNP_ERR;
     d.nsg_rows = (int)rows.size();
     // the descriptor lists depend on the coordinates only: they outlive a

@@ -90,12 +90,13 @@ __global__ __launch_bounds__(kEpi) void k_born(Dev d) {
       // surrogate call: the model's sum plus the correction of the last
       // anchor; anchor call: record the surrogate's error and re-anchor
       const int mode = d.sg_state[4 * d.asys[i] + 2];
-      const double pred = (double)d.sg_s[i] + d.sg_c[i];
+      const double sm = (double)d.sg_s[d.sg_slot[i]];      // the model's output is in slot order
+      const double pred = sm + d.sg_c[i];
       if (mode == 1) {
         s = pred;
       } else {
         d.sg_delta[i] = pred - s;
-        d.sg_c[i] = s - (double)d.sg_s[i];
+        d.sg_c[i] = s - sm;
         d.sg_target[i] = (float)s;
       }
     }

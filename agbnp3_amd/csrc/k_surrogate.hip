@@ -124,6 +124,7 @@ __global__ __launch_bounds__(1024) void k_sg_cells(Dev d) {
   __syncthreads();
   bin_atoms(g, d.pos, S.a0, S.n, d.sg_ctmp, d.sg_cstart + (size_t)b * (kMaxCells + 1), d.sg_catoms, cnt, wsum, wpre,
             true);
+  for (int k = threadIdx.x; k < S.n; k += 1024) d.sg_slot[d.sg_catoms[S.a0 + k]] = S.a0 + k;
 }
 
 // one wavefront per row (four per block, over one row block): the cells
@@ -156,10 +157,9 @@ __global__ __launch_bounds__(256) void k_sg_nbl(Dev d) {
         const int beg = cstart[c0 + lo[0]], end = cstart[c0 + hi[0] + 1];
         for (int k0 = beg; k0 < end; k0 += 64) {
           const int k = k0 + lane;
-          int j = -1;
           bool th = false, tH = false;
-          if (k < end) {
-            j = d.sg_catoms[k];
+          if (k < end) {                             // k: the candidate's slot
+            const int j = d.sg_catoms[k];
             const float4 pj = d.pos[j];
             const float ex = pj.x - pa.x, ey = pj.y - pa.y, ez = pj.z - pa.z;
             const bool in = ex * ex + ey * ey + ez * ez < R2 && j != a;
@@ -168,12 +168,12 @@ __global__ __launch_bounds__(256) void k_sg_nbl(Dev d) {
           }
           const unsigned long long mh = __ballot(th), mH = __ballot(tH);
           if (th) {
-            const int slot = nh + __popcll(mh & below);
-            if (slot < kSgNbrCap) nb[slot] = j;
+            const int e = nh + __popcll(mh & below);
+            if (e < kSgNbrCap) nb[e] = k;
           }
           if (tH) {
-            const int slot = kSgNbrCap - 1 - (nH + __popcll(mH & below));
-            if (slot >= 0) nb[slot] = j;
+            const int e = kSgNbrCap - 1 - (nH + __popcll(mH & below));
+            if (e >= 0) nb[e] = k;
           }
           nh += __popcll(mh);
           nH += __popcll(mH);
@@ -189,64 +189,83 @@ __global__ __launch_bounds__(256) void k_sg_nbl(Dev d) {
   if (threadIdx.x == 0) d.sg_nbl[4 * sys + 0] = 1;
 }
 
+// per call: positions and self-volume fractions in slot (cell) order, so that
+// a row's neighbours, listed in cell order, are read from nearby addresses
+__global__ __launch_bounds__(256) void k_sg_shadow(Dev d) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= d.n || !d.sys_sg[d.asys[t]]) return;
+  const int a = d.sg_catoms[t];
+  const SysInfo &S = d.sys[d.asys[t]];
+  d.sg_spos[t] = d.pos[a];
+  d.sg_sgam[t] = a < S.a0 + S.nh ? d.sp_f[a] : 0.f;
+}
+// and the weights of the gradient pass (after q2ab, abrw are known)
+__global__ __launch_bounds__(256) void k_sg_shadow_w(Dev d) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= d.n || !d.sys_sg[d.asys[t]]) return;
+  const int a = d.sg_catoms[t];
+  d.sg_sw[t] = make_float2(d.q2ab_f[a], d.abrw_f[a]);
+}
+
 // ------------------------------------------------------------ descriptors
-// one wavefront per row, its lanes over the row's heavy neighbours; the 64
-// shell sums of the row are LDS bins of the wave (each pair adds to the 4
-// shells its B-spline covers, LDS float atomics), then lane m writes bin m
+// one wavefront per row (a slot), its lanes over the row's heavy neighbours;
+// the 64 shell sums of the row are LDS bins of the wave, each pair adding to
+// the 4 shells its B-spline covers.  The bins hold fixed point (2^-24, sums
+// below 128) and take integer atomics: LDS float atomics run far slower on
+// this GPU, and integer sums do not depend on the order of the adds.
+constexpr float kSgBinScale = 16777216.0f;      // 2^24
 __global__ __launch_bounds__(256) void k_sg_features(Dev d) {
-  __shared__ float bins[4][64];
+  __shared__ unsigned bins[4][64];
   const int rb = xcd_block(blockIdx.x, gridDim.x);
   const int2 blk = d.sg_rows[rb];
   const int sys = d.asys[blk.x];
   if (!d.sys_sg[sys]) return;       // no model on this system (block-uniform)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool over = d.sg_nbl[4 * sys + 3] != 0;
-  float *bin = bins[w];
+  unsigned *bin = bins[w];
   for (int r = w; r < blk.y - blk.x; r += 4) {
-    const int i = d.sg_catoms[blk.x + r];
-    const float4 pi = d.pos[i];
+    const int t = blk.x + r;                   // the row's slot
+    const int i = d.sg_catoms[t];
+    const float4 pi = d.sg_spos[t];
     const int nh = over ? 0 : d.sg_nn[i].x;
     const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
-    bin[lane] = 0.f;
-    // each lane takes its own stretch of the list (neighbouring entries are
-    // near each other and would add to the same bins at the same time)
-    const int chunk = (nh + 63) >> 6;
-    for (int st = 0; st < chunk; st++) {
-      const int k = lane * chunk + st;
+    bin[lane] = 0u;
+    for (int k0 = 0; k0 < nh; k0 += 64) {
+      const int k = k0 + lane;
       if (k >= nh) continue;
-      const int j = nb[k];
-      const float4 pj = d.pos[j];
+      const int u = nb[k];
+      const float4 pj = d.sg_spos[u];
       const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const float d2 = ex * ex + ey * ey + ez * ez;
       if (!(d2 < kSgCut * kSgCut)) continue;
       const float dist = d2 * rsqrtf(d2);
       float T, Tp, b[4], db[4];
       sg_taper(dist, T, Tp);
-      T *= d.sp_f[j];
+      T *= d.sg_sgam[u] * kSgBinScale;
       const int k0s = sg_basis(dist, b, db);
-      float *f = bin + sg_channel(pj.w) * 16;
+      unsigned *f = bin + sg_channel(pj.w) * 16;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int kk = k0s + q;
-        if (kk >= 0 && kk < kSgNShell) atomicAdd(&f[kk], b[q] * T);
+        if (kk >= 0 && kk < kSgNShell) atomicAdd(&f[kk], (unsigned)__float2uint_rn(b[q] * T));
       }
     }
-    // raw shell sums, normalised input row, in-range test
-    const float raw = bin[lane];
-    d.sg_F[(size_t)i * 64 + lane] = raw;
+    const float raw = (float)bin[lane] * (1.0f / kSgBinScale);
+    // raw shell sums, normalised input row, in-range test (slot order)
+    d.sg_F[(size_t)t * 64 + lane] = raw;
     float v = (raw - d.sg.xmu[lane]) * d.sg.xsd_inv[lane];
     bool oor = !(v >= d.sg.xlo[lane] && v <= d.sg.xhi[lane]);
-    float *x = d.sg_X + (size_t)i * kSgKPad;
+    float *x = d.sg_X + (size_t)t * kSgKPad;
     x[lane] = v;
     if (lane < kSgKPad - 64) {                    // R, q, then zero padding
       const int k = 64 + lane;
-      float u = 0.f;
+      float uu = 0.f;
       if (k < kSgNFeat) {
         const float rw = k == 64 ? pi.w : d.q[i];
-        u = (rw - d.sg.xmu[k]) * d.sg.xsd_inv[k];
-        oor |= !(u >= d.sg.xlo[k] && u <= d.sg.xhi[k]);
+        uu = (rw - d.sg.xmu[k]) * d.sg.xsd_inv[k];
+        oor |= !(uu >= d.sg.xlo[k] && uu <= d.sg.xhi[k]);
       }
-      x[k] = u;
+      x[k] = uu;
     }
     if (__any(oor || over)) {
       if (lane == 0) d.sg_state[4 * sys + 3] = 1;
@@ -423,7 +442,8 @@ __global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
     mlp_backward_tile(m, L, true, [](int) {});
     for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
       const int r = k >> 6, c = k & 63;
-      if (r < nrows) d.sg_dsdF[(size_t)(r0 + r) * 64 + c] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
+      if (r < nrows)                  // channel-major: a row's neighbours read adjacent 64-byte slices
+        d.sg_dsdF[((size_t)(c >> 4) * d.n + r0 + r) * 16 + (c & 15)] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
     }
   }
 }
@@ -623,24 +643,35 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   float *Ga = Gs[w];
   for (int r = w; r < blk.y - blk.x; r += 4) {
-    const int a = d.sg_catoms[blk.x + r];
-    const float4 pa = d.pos[a];
+    const int t = blk.x + r;                   // the row's slot
+    const int a = d.sg_catoms[t];
+    const float4 pa = d.sg_spos[t];
     const int2 nn = d.sg_nn[a];
     const bool aheavy = a < S.a0 + S.nh;
     const int cha = aheavy ? sg_channel(pa.w) : 0;
-    const float gama = aheavy ? d.sp_f[a] : 0.f;
-    const float *dsdF = d.sg_dsdF + cha * 16;
+    const float gama = d.sg_sgam[t];
+    const float *dsdF = d.sg_dsdF + (size_t)cha * d.n * 16;     // channel cha of every slot
     const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
-    Ga[lane] = d.sg_dsdF[(size_t)a * 64 + lane];
+    Ga[lane] = d.sg_dsdF[((size_t)(lane >> 4) * d.n + t) * 16 + (lane & 15)];
     float sx = 0.f, sy = 0.f, sz = 0.f;                 // d s_a / d x_a
     float gx = 0.f, gy = 0.f, gz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;   // neighbour terms
     float ru = 0.f, rv = 0.f;                           // sum_i {q2ab_i, abrw_i} d s_i / d g_a
-    // heavy neighbours: both terms (the second for heavy rows)
-    for (int k0 = 0; k0 < nn.x; k0 += 64) {
-      const int k = k0 + lane;
-      if (k >= nn.x) continue;
-      const int j = nb[k];
-      const float4 pj = d.pos[j];
+    // one stream over the list: heavy neighbours (entries 0 .. nn.x-1; both
+    // terms, the second for heavy rows), then, for heavy rows, hydrogens
+    // (entries from the top; the second term only).  The next step's slot
+    // and position are loaded one step ahead.
+    const int ntot = nn.x + (aheavy ? nn.y : 0);
+    auto entry = [&](int v) { return nb[v < nn.x ? v : kSgNbrCap - 1 - (v - nn.x)]; };
+    int jn = lane < ntot ? entry(lane) : t;
+    float4 pn = d.sg_spos[jn];
+    for (int v0 = 0; v0 < ntot; v0 += 64) {
+      const int v = v0 + lane;
+      const int j = jn;                        // the neighbour's slot
+      const float4 pj = pn;
+      jn = v + 64 < ntot ? entry(v + 64) : t;
+      pn = d.sg_spos[jn];
+      if (v >= ntot) continue;
+      const bool jheavy = v < nn.x;
       const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;   // x_a - x_j
       const float d2 = ex * ex + ey * ey + ez * ez;
       if (!(d2 < kSgCut * kSgCut)) continue;
@@ -649,24 +680,27 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
       sg_taper(dist, T, Tp);
       const int ks = sg_basis(dist, b, db);
       const float *Gc = Ga + sg_channel(pj.w) * 16;
-      const float *Gj = dsdF + (size_t)j * 64;
+      const float *Gj = dsdF + (size_t)j * 16;
       float cA = 0.f, cB = 0.f, cU = 0.f;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int kk = ks + q;
         if (kk < 0 || kk >= kSgNShell) continue;
         const float dphi = db[q] * T + b[q] * Tp;
-        cA += Gc[kk] * dphi;
+        if (jheavy) cA += Gc[kk] * dphi;
         if (aheavy) {
           const float g = Gj[kk];
           cB += g * dphi;
           cU += g * b[q];
         }
       }
-      cA *= d.sp_f[j] * inv;          // the neighbour's weight in the shell sums
-      sx += cA * ex; sy += cA * ey; sz += cA * ez;
+      if (jheavy) {
+        cA *= d.sg_sgam[j] * inv;     // the neighbour's weight in the shell sums
+        sx += cA * ex; sy += cA * ey; sz += cA * ez;
+      }
       if (aheavy) {
-        const float wq = d.q2ab_f[j], wa = d.abrw_f[j];
+        const float2 wj = d.sg_sw[j];
+        const float wq = wj.x, wa = wj.y;
         cB *= gama * inv;
         cU *= T;
         gx += wq * cB * ex; gy += wq * cB * ey; gz += wq * cB * ez;
@@ -675,44 +709,14 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
         rv += wa * cU;
       }
     }
-    // hydrogen neighbours of heavy rows: the second term only
-    for (int k0 = 0; k0 < nn.y; k0 += 64) {
-      const int k = k0 + lane;
-      if (k >= nn.y) continue;
-      const int j = nb[kSgNbrCap - 1 - k];
-      const float4 pj = d.pos[j];
-      const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;
-      const float d2 = ex * ex + ey * ey + ez * ez;
-      if (!(d2 < kSgCut * kSgCut)) continue;
-      const float inv = rsqrtf(d2), dist = d2 * inv;
-      float T, Tp, b[4], db[4];
-      sg_taper(dist, T, Tp);
-      const int ks = sg_basis(dist, b, db);
-      const float *Gj = dsdF + (size_t)j * 64;
-      float cB = 0.f, cU = 0.f;
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int kk = ks + q;
-        if (kk < 0 || kk >= kSgNShell) continue;
-        const float g = Gj[kk];
-        cB += g * (db[q] * T + b[q] * Tp);
-        cU += g * b[q];
-      }
-      const float wq = d.q2ab_f[j], wa = d.abrw_f[j];
-      cB *= gama * inv;
-      cU *= T;
-      gx += wq * cB * ex; gy += wq * cB * ey; gz += wq * cB * ez;
-      vx += wa * cB * ex; vy += wa * cB * ey; vz += wa * cB * ez;
-      ru += wq * cU;
-      rv += wa * cU;
-    }
     // fixed-order wave sums
     sx = wave_sum(sx); sy = wave_sum(sy); sz = wave_sum(sz);
     gx = wave_sum(gx); gy = wave_sum(gy); gz = wave_sum(gz);
     vx = wave_sum(vx); vy = wave_sum(vy); vz = wave_sum(vz);
     ru = wave_sum(ru); rv = wave_sum(rv);
     if (lane == 0) {
-      const float qa = d.q2ab_f[a], ba = d.abrw_f[a];
+      const float2 wa2 = d.sg_sw[t];
+      const float qa = wa2.x, ba = wa2.y;
       long long *p = d.acc_ch + 8 * (size_t)a;
       fix_add(p + 0, -(double)(qa * sx + gx));
       fix_add(p + 1, -(double)(qa * sy + gy));
@@ -761,7 +765,8 @@ __global__ __launch_bounds__(256) void k_sg_append(Dev d, float *F, float *targe
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= d.n) return;
   float *f = F + (size_t)(at + i) * kSgNFeat;
-  for (int k = 0; k < 64; k++) f[k] = d.sg_F[(size_t)i * 64 + k];
+  const float *src = d.sg_F + (size_t)d.sg_slot[i] * 64;
+  for (int k = 0; k < 64; k++) f[k] = src[k];
   f[64] = d.R[i];
   f[65] = d.q[i];
   target[at + i] = d.sg_target[i];
@@ -776,6 +781,7 @@ void launch_sg_front(const Dev &d, hipStream_t s) {
   hipLaunchKernelGGL(k_sg_guard, dim3(d.nsys), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_cells, dim3(d.nsys), dim3(1024), 0, s, d);
   hipLaunchKernelGGL(k_sg_nbl, dim3(d.nsg_rows), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_sg_shadow, dim3((d.n + 255) / 256), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_features, dim3(d.nsg_rows), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_mode, dim3((d.nsys + 63) / 64), dim3(64), 0, s, d);
   const int ntiles = (d.n + kSgTile - 1) / kSgTile;
@@ -841,6 +847,7 @@ void launch_sg_infer(const SgModel &m, const float *X, int n, float *s_out, floa
 
 void launch_sg_grad(const Dev &d, hipStream_t s) {
   if (!d.sg_on) return;
+  hipLaunchKernelGGL(k_sg_shadow_w, dim3((d.n + 255) / 256), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_grad, dim3(d.nsg_rows), dim3(256), 0, s, d);
 }
 
