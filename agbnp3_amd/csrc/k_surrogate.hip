@@ -40,9 +40,11 @@ constexpr int kSgTile = 32;                 // atoms per MLP tile
 constexpr int kLd1 = kSgKPad + 8;           // LDS row strides (bf16 elements), padded
 constexpr int kLdH = kSgHid + 8;
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid with the hardware reciprocal (1 ulp; exp overflow gives 0)
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float silup(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigm(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -299,22 +301,61 @@ __device__ __forceinline__ void mlp_layer(const __bf16 *Ahi, const __bf16 *Alo, 
   }
 }
 
-// Out[32][NOUT] = D[32][128] W^T with W row-major [NOUT rows][128] in global memory
-template <int NOUT>
-__device__ __forceinline__ void mlp_back(const __bf16 *D, const unsigned short *W, float *Out) {
+// The backward products Out[32][NOUT] = D[32][128] W^T (W row-major, NOUT
+// rows of 128) need, in each wave, the B operands of the same output tiles
+// for every 32-atom tile: they are loaded once per block into registers
+// (2 x 4 bf16x8 for each 128-wide layer, 4 for the 64-wide first layer).
+struct BackW {
+  bf16x8 w3[2][4], w2[2][4], w1[4];
+};
+__device__ __forceinline__ bf16x8 back_operand(const unsigned short *W, int nt, int kk) {
+  const int lane = threadIdx.x & 63;
+  return *(const bf16x8 *)(W + (size_t)(nt * 16 + (lane & 15)) * kSgHid + kk * 32 + 8 * (lane >> 4));
+}
+__device__ __forceinline__ void load_backw(const SgModel &m, BackW &B) {
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      B.w3[i][kk] = back_operand(m.w3, w + 4 * i, kk);
+      B.w2[i][kk] = back_operand(m.w2, w + 4 * i, kk);
+    }
+    B.w1[kk] = back_operand(m.w1, w, kk);
+  }
+}
+// 128 outputs: this wave's column tiles w and w + 4, both row tiles
+__device__ __forceinline__ void mlp_back128(const __bf16 *D, const bf16x8 (&Bw)[2][4], float *Out) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  constexpr int NT = NOUT / 16;
-  for (int t = w; t < 2 * NT; t += 4) {
-    const int mt = t / NT, nt = t % NT;
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++) {
+    bf16x8 a[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) a[kk] = *(const bf16x8 *)(D + (mt * 16 + (lane & 15)) * kLdH + kk * 32 + 8 * (lane >> 4));
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int nt = w + 4 * i;
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kk], Bw[i][kk], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) Out[(mt * 16 + 4 * (lane >> 4) + r) * kSgHid + nt * 16 + (lane & 15)] = acc[r];
+    }
+  }
+}
+// 64 outputs (the shell features): this wave's column tile w, both row tiles
+__device__ __forceinline__ void mlp_back64(const __bf16 *D, const bf16x8 (&Bw)[4], float *Out) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++) {
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k0 = 0; k0 < kSgHid; k0 += 32) {
-      const bf16x8 a = *(const bf16x8 *)(D + (mt * 16 + (lane & 15)) * kLdH + k0 + 8 * (lane >> 4));
-      const bf16x8 b = *(const bf16x8 *)(W + (size_t)(nt * 16 + (lane & 15)) * kSgHid + k0 + 8 * (lane >> 4));
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    for (int kk = 0; kk < 4; kk++) {
+      const bf16x8 a = *(const bf16x8 *)(D + (mt * 16 + (lane & 15)) * kLdH + kk * 32 + 8 * (lane >> 4));
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, Bw[kk], acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) Out[(mt * 16 + 4 * (lane >> 4) + r) * kSgHid + nt * 16 + (lane & 15)] = acc[r];
+    for (int r = 0; r < 4; r++) Out[(mt * 16 + 4 * (lane >> 4) + r) * kSgHid + w * 16 + (lane & 15)] = acc[r];
   }
 }
 
@@ -402,22 +443,22 @@ __device__ void mlp_seed_delta3(const SgModel &m, MlpLds &L) {
 // delta is first offered to keep(layer, act) (training stores them); the
 // input gradient d y / d X (first 64 columns) is left in z3 when `want_dx`.
 template <typename Keep>
-__device__ void mlp_backward_tile(const SgModel &m, MlpLds &L, bool want_dx, Keep keep) {
+__device__ void mlp_backward_tile(const BackW &B, MlpLds &L, bool want_dx, Keep keep) {
   keep(3);
-  mlp_back<kSgHid>(L.act, m.w3, L.z3);
+  mlp_back128(L.act, B.w3, L.z3);
   __syncthreads();
   for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
     L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)(L.z3[k] * silup(L.z2[k]));
   __syncthreads();
   keep(2);
-  mlp_back<kSgHid>(L.act, m.w2, L.z3);
+  mlp_back128(L.act, B.w2, L.z3);
   __syncthreads();
   for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256)
     L.act[(k / kSgHid) * kLdH + (k % kSgHid)] = (__bf16)(L.z3[k] * silup(L.z1[k]));
   __syncthreads();
   keep(1);
   if (want_dx) {
-    mlp_back<64>(L.act, m.w1, L.z3);
+    mlp_back64(L.act, B.w1, L.z3);
     __syncthreads();
   }
 }
@@ -428,6 +469,8 @@ __global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
   MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
   const SgModel &m = d.sg;
   stage_weights(m, L);
+  BackW BW;
+  load_backw(m, BW);
   const int ntiles = (d.n + kSgTile - 1) / kSgTile;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int r0 = tile * kSgTile, nrows = min(kSgTile, d.n - r0);
@@ -439,7 +482,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
     }
     __syncthreads();
     mlp_seed_delta3(m, L);
-    mlp_backward_tile(m, L, true, [](int) {});
+    mlp_backward_tile(BW, L, true, [](int) {});
     for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
       const int r = k >> 6, c = k & 63;
       if (r < nrows)                  // channel-major: a row's neighbours read adjacent 64-byte slices
@@ -458,6 +501,8 @@ __global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r
   extern __shared__ __align__(16) unsigned char lds_raw[];
   MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
   stage_weights(m, L);
+  BackW BW;
+  load_backw(m, BW);
   const int ntiles = (B + kSgTile - 1) / kSgTile;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int b0 = tile * kSgTile, nrows = min(kSgTile, B - b0);
@@ -492,7 +537,7 @@ __global__ __launch_bounds__(256) void k_sg_train_fb(SgModel m, SgTrain t, int r
         t.A0[(size_t)c * B + b0 + r] = __builtin_bit_cast(unsigned short, (__bf16)t.X[(size_t)(r0 + b0 + r) * kSgKPad + c]);
     }
     mlp_seed_delta3(m, L);
-    mlp_backward_tile(m, L, false, [&](int layer) {
+    mlp_backward_tile(BW, L, false, [&](int layer) {
       unsigned short *D = layer == 3 ? t.D3 : (layer == 2 ? t.D2 : t.D1);
       for (int k = threadIdx.x; k < kSgTile * kSgHid; k += 256) {
         const int r = k / kSgHid, c = k % kSgHid;
@@ -821,6 +866,8 @@ __global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const float *X, int
   extern __shared__ __align__(16) unsigned char lds_raw[];
   MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
   stage_weights(m, L);
+  BackW BW;
+  load_backw(m, BW);
   const int ntiles = (n + kSgTile - 1) / kSgTile;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int r0 = tile * kSgTile, nrows = min(kSgTile, n - r0);
@@ -832,7 +879,7 @@ __global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const float *X, int
     }
     __syncthreads();
     mlp_seed_delta3(m, L);
-    mlp_backward_tile(m, L, true, [](int) {});
+    mlp_backward_tile(BW, L, true, [](int) {});
     for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
       const int r = k >> 6, c = k & 63;
       if (r < nrows) dsdF[(size_t)(r0 + r) * 64 + c] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
