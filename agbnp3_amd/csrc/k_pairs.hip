@@ -266,8 +266,6 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
     const float iRk = 1.0f / pk.w;
     __syncthreads();
     float rg[3] = {0.f, 0.f, 0.f}, rv[3] = {0.f, 0.f, 0.f}, rU = 0.f, rV = 0.f;
-    float cacc[4][8];                                // this wave's column sums, per 64-column chunk
-#pragma unroll
     for (int c = 0; c < 4; c++) {
       float ca[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       for (int s = 0; s < 64; s++) {
@@ -311,8 +309,14 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
 #pragma unroll
         for (int q = 0; q < 8; q++) ca[q] = wave_rot(ca[q]);
       }
+      // this chunk's column sums over the four waves in a fixed order (0 + 1 + 2 + 3)
+      for (int ww = 0; ww < 4; ww++) {
+        if (w == ww) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) cacc[c][q] = ca[q];
+          for (int q = 0; q < 8; q++) colacc[q][c * 64 + lane] = ww == 0 ? ca[q] : colacc[q][c * 64 + lane] + ca[q];
+        }
+        __syncthreads();
+      }
     }
     if (kr >= 0) {
       long long *pr = d.acc_ch + 8 * (size_t)kr;
@@ -322,17 +326,6 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
       }
       fix_add(pr + 6, (double)rU);
       fix_add(pr + 7, (double)rV);
-    }
-    // column sums over the four waves in a fixed order (wave 0 + 1 + 2 + 3)
-    for (int ww = 0; ww < 4; ww++) {
-      if (w == ww) {
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-#pragma unroll
-          for (int q = 0; q < 8; q++)
-            colacc[q][c * 64 + lane] = ww == 0 ? cacc[c][q] : colacc[q][c * 64 + lane] + cacc[c][q];
-      }
-      __syncthreads();
     }
     const int j = d.cperm[item.y * 256 + threadIdx.x];
     if (!diag && j >= 0) {
