@@ -112,9 +112,9 @@ void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s) {
 // a DPP wave rotation, so every lane meets every column once without LDS
 // traffic.  Diagonal tiles see each pair twice and keep only the row side.
 __device__ __forceinline__ float wave_rot(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, true));
 }
-__device__ __forceinline__ int wave_rot(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, false); }
+__device__ __forceinline__ int wave_rot(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, true); }
 
 __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
   __shared__ float colacc[4][4][256];   // [wave][A, gx, gy, gz][column]: fixed-order sum
