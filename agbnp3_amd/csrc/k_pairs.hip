@@ -146,12 +146,11 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
           cq = d.q[j0];
           cb = d.br_f[j0];
         }
-        int cj = j0;
         for (int s = 0; s < 64; s++) {
           const float ex = cx - xi, ey = cy - yi, ez = cz - zi;
           const float d2 = ex * ex + ey * ey + ez * ez;
-          const bool valid = (cj != i) && (d2 > 0.f);
-          const float qq = valid ? qi * cq : 0.f;
+          // the reference's mask (kernels.c:107): d2 > 0, which also drops the self pair
+          const float qq = d2 > 0.f ? qi * cq : 0.f;
           const float bb = bi * cb;
           const float et = __expf(-0.25f * d2 * __builtin_amdgcn_rcpf(bb));
           const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
@@ -164,7 +163,6 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
           cA += at;
           cgx -= mw * ex; cgy -= mw * ey; cgz -= mw * ez;
           cx = wave_rot(cx); cy = wave_rot(cy); cz = wave_rot(cz); cq = wave_rot(cq); cb = wave_rot(cb);
-          cj = wave_rot(cj);
           cA = wave_rot(cA); cgx = wave_rot(cgx); cgy = wave_rot(cgy); cgz = wave_rot(cgz);
         }
         // 64 rotations bring every column's sums back to its starting lane
