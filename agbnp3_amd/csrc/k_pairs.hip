@@ -78,7 +78,7 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
         for (int c = 0; c < nc; c++) {                // independent columns overlap; sums stay in order
           const float4 pj = cpos[c];
           const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-          const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+          const float dist = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);   // hardware square root (1 ulp)
           const float2 w = cw[c];
           float q, dq;
           q4_lookup<true, false>(tab[__float_as_int(w.y)], dist * pj.w, dx_, dxinv, dp1, dp2, 1.f, q, dq);
