@@ -64,10 +64,10 @@ __device__ __forceinline__ OvlOut overlap(const Gauss &par, float ab, float xb, 
   const float dx = xb - par.x, dy = yb - par.y, dz = zb - par.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
   const float a12 = par.a + ab;
-  const float dinv = 1.0f / a12;
-  const float p12 = par.p * kPFC * expf(-par.a * ab * d2 * dinv);
+  const float dinv = __builtin_amdgcn_rcpf(a12);                    // hardware reciprocal, exp, sqrt (1-2 ulp)
+  const float p12 = par.p * kPFC * __expf(-par.a * ab * d2 * dinv);
   const float u = kPiF * dinv;
-  o.V = p12 * u * sqrtf(u);
+  o.V = p12 * u * __builtin_amdgcn_sqrtf(u);
   float s, sp, spp;
   vol_switch(o.V, s, sp, spp);
   o.Vp = s * o.V;
@@ -86,7 +86,7 @@ __device__ __forceinline__ OvlOut overlap(const Gauss &par, float ab, float xb, 
 __device__ __forceinline__ float sa_term(const OvlOut &o, float sign, float a, float x, float y, float z, float R) {
   const float ex = x - o.cx, ey = y - o.cy, ez = z - o.cz;
   const float t = ex * ex + ey * ey + ez * ez;
-  return sign * o.fp * a * o.V * (3.f * o.dinv + 2.f * t) / R;
+  return sign * o.fp * a * o.V * (3.f * o.dinv + 2.f * t) * __builtin_amdgcn_rcpf(R);
 }
 
 // Contributions of one kept overlap o of order N with members mem[0..N-1]
@@ -120,7 +120,7 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
     const float Uc = -df * fourpi1 * cn * Us * o.fp;
     const float Vc = -fourpi1 * cn * Vs * o.fp;
     const float Hc = -cn * Hs * o.fp;
-    const float h3 = o.fpp + o.fp / o.V, h4 = 2.f * o.fp * o.dinv;
+    const float h3 = o.fpp + o.fp * __builtin_amdgcn_rcpf(o.V), h4 = 2.f * o.fp * o.dinv;
     float gdr[N][3], gdR[N];
     float S1c = 0.f, S1v = 0.f, S1g = 0.f;
     float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
@@ -131,9 +131,9 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
       gdr[k][0] = -2.f * av * ex;
       gdr[k][1] = -2.f * av * ey;
       gdr[k][2] = -2.f * av * ez;
-      gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) / R_[k];
+      gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) * __builtin_amdgcn_rcpf(R_[k]);
       S1c += WC[k] * gdR[k]; S1v += WV[k] * gdR[k]; S1g += WG[k] * gdR[k];
-      const float iR = 1.0f / R_[k];
+      const float iR = __builtin_amdgcn_rcpf(R_[k]);
 #pragma unroll
       for (int q = 0; q < 3; q++) {
         S2c[q] += WC[k] * gdr[k][q] * iR;
@@ -143,7 +143,7 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
     }
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      const float iR = 1.0f / R_[k];
+      const float iR = __builtin_amdgcn_rcpf(R_[k]);
       const float t = -2.f * o.fp * (1.f - A_[k] * o.dinv) * iR;
       const float bc = h3 * S1c + WC[k] * t, bv = h3 * S1v + WV[k] * t, bg = h3 * S1g + WG[k] * t;
       const float ha = h4 * A_[k];
@@ -237,7 +237,7 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
         const float Uc = -df * fourpi1 * cn * Us * o.fp;
         const float Vc = -fourpi1 * cn * Vs * o.fp;
         const float Hc = -cn * Hs * o.fp;
-        const float h3 = o.fpp + o.fp / o.V, h4 = 2.f * o.fp * o.dinv;
+        const float h3 = o.fpp + o.fp * __builtin_amdgcn_rcpf(o.V), h4 = 2.f * o.fp * o.dinv;
         float gdr[N][3], gdR[N];
         float S1c = 0.f, S1v = 0.f, S1g = 0.f;
         float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
@@ -248,9 +248,9 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
           gdr[k][0] = -2.f * av * ex;
           gdr[k][1] = -2.f * av * ey;
           gdr[k][2] = -2.f * av * ez;
-          gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) / R_[k];
+          gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) * __builtin_amdgcn_rcpf(R_[k]);
           S1c += WC[k] * gdR[k]; S1v += WV[k] * gdR[k]; S1g += WG[k] * gdR[k];
-          const float iR = 1.0f / R_[k];
+          const float iR = __builtin_amdgcn_rcpf(R_[k]);
 #pragma unroll
           for (int q = 0; q < 3; q++) {
             S2c[q] += WC[k] * gdr[k][q] * iR;
@@ -260,7 +260,7 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
         }
 #pragma unroll
         for (int k = 0; k < N; k++) {
-          const float iR = 1.0f / R_[k];
+          const float iR = __builtin_amdgcn_rcpf(R_[k]);
           const float t = -2.f * o.fp * (1.f - A_[k] * o.dinv) * iR;
           const float bc = h3 * S1c + WC[k] * t, bv = h3 * S1v + WV[k] * t, bg = h3 * S1g + WG[k] * t;
           const float ha = h4 * A_[k];
