@@ -89,6 +89,15 @@ __device__ __forceinline__ float sa_term(const OvlOut &o, float sign, float a, f
   return sign * o.fp * a * o.V * (3.f * o.dinv + 2.f * t) * __builtin_amdgcn_rcpf(R);
 }
 
+// add to a per-slot accumulator: FP64 LDS atomics (one wave per root: the
+// lanes' order is fixed), or 64-bit fixed point with integer atomics (several
+// waves per root: the sum does not depend on their order)
+__device__ __forceinline__ void slot_add(double *p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void slot_add(long long *p, double v) {
+  const long long q = __double2ll_rn(v * kFix);
+  if (q != 0) atomicAdd((unsigned long long *)p, (unsigned long long)q);
+}
+
 // Contributions of one kept overlap o of order N with members mem[0..N-1]
 // (mem[0] = 0, the root) to the root's registers and the slot accumulators.
 template <int N, int CAP, bool PB, typename Lds>
@@ -103,8 +112,8 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
 #pragma unroll
     for (int k = 1; k < N; k++) {
       const int sk = mem[k];
-      atomicAdd(&S.acc[0][sk], wv);
-      atomicAdd(&S.acc[1][sk], (double)sa_term(o, sign, S.a[sk], S.x[sk], S.y[sk], S.z[sk], S.R[sk]));
+      slot_add(&S.acc[0][sk], wv);
+      slot_add(&S.acc[1][sk], (double)sa_term(o, sign, S.a[sk], S.x[sk], S.y[sk], S.z[sk], S.R[sk]));
     }
   } else {
     float A_[N], X_[N], Y_[N], Z_[N], R_[N], WC[N], WV[N], WG[N];
@@ -161,7 +170,7 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
         for (int v = 0; v < TG_N; v++) root[v] += (double)out[v];
       } else {
 #pragma unroll
-        for (int v = 0; v < TG_N; v++) atomicAdd(&S.acc[v][mem[k]], (double)out[v]);
+        for (int v = 0; v < TG_N; v++) slot_add(&S.acc[v][mem[k]], (double)out[v]);
       }
     }
   }
@@ -217,11 +226,11 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
         root[1] += (double)sa_term(o, sign, ma[0], mx[0], my[0], mz[0], mR[0]);
 #pragma unroll
         for (int k = 1; k < N - 1; k++) {
-          atomicAdd(&S.acc[0][path[k]], wv);
-          atomicAdd(&S.acc[1][path[k]], (double)sa_term(o, sign, ma[k], mx[k], my[k], mz[k], mR[k]));
+          slot_add(&S.acc[0][path[k]], wv);
+          slot_add(&S.acc[1][path[k]], (double)sa_term(o, sign, ma[k], mx[k], my[k], mz[k], mR[k]));
         }
-        atomicAdd(&S.acc[0][sb], wv);
-        atomicAdd(&S.acc[1][sb], (double)sa_term(o, sign, ab, xb, yb, zb, Rb));
+        slot_add(&S.acc[0][sb], wv);
+        slot_add(&S.acc[1][sb], (double)sa_term(o, sign, ab, xb, yb, zb, Rb));
       } else {
         // members: path[0..N-2] then the candidate (index N-1)
         float A_[N], X_[N], Y_[N], Z_[N], R_[N], WC[N], WV[N], WG[N];
@@ -279,7 +288,7 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
           } else {
             const int slot = k < N - 1 ? path[k] : sb;
 #pragma unroll
-            for (int v = 0; v < TG_N; v++) atomicAdd(&S.acc[v][slot], (double)out[v]);
+            for (int v = 0; v < TG_N; v++) slot_add(&S.acc[v][slot], (double)out[v]);
           }
         }
       }
@@ -389,20 +398,58 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
 constexpr int kLevelCap = 128;          // nodes of one order held in LDS
 constexpr int kLevelCapBig = 1536;      // second tier (dynamic LDS) for the largest trees
 
-template <int CAP, bool PB, int LC>
+// slot accumulators: FP64 for one wave per root, fixed point for several
+template <int W> struct SlotAcc { using T = long long; };
+template <> struct SlotAcc<1> { using T = double; };
+
+template <int CAP, bool PB, int LC, int W>
 struct BfsLds {
   int idx[CAP + 1];
   float a[CAP + 1], x[CAP + 1], y[CAP + 1], z[CAP + 1], R[CAP + 1];
   float u[PB ? CAP + 1 : 1], v[PB ? CAP + 1 : 1], h[PB ? CAP + 1 : 1];
   float wc[PB ? CAP + 1 : 1], wv[PB ? CAP + 1 : 1], wg[PB ? CAP + 1 : 1];
-  double acc[PB ? TG_N : 2][CAP + 1];
+  typename SlotAcc<W>::T acc[PB ? TG_N : 2][CAP + 1];
   // two node buffers: Gaussian, member slots (16 bits each, root excluded),
   // end of the node's sibling group, parent index in the previous order
   float ga[2][LC], gp[2][LC], gx[2][LC], gy[2][LC], gz[2][LC];
   unsigned long long path[2][LC];
   int gend[2][LC], gpar[2][LC];
   int pref[LC + 1], cnt[LC];
+  int wcnt[2][W];                       // kept candidates per wave of one pass
 };
+
+// Compaction of one pass over W*64 candidates (candidate = pass base +
+// 64 * wave + lane): this lane's position among the pass's kept candidates,
+// and their total (block-uniform).  wcnt alternates between two rows by
+// pass parity, so one barrier per pass separates its writes from the reads
+// of the pass before.
+template <int W>
+__device__ __forceinline__ int pass_compact(int (&wcnt)[2][W], int parity, unsigned long long bal, int &total) {
+  const int lane = threadIdx.x & 63;
+  const int below = __popcll(bal & ((1ull << lane) - 1ull));
+  if (W == 1) {
+    total = __popcll(bal);
+    return below;
+  }
+  const int w = threadIdx.x >> 6;
+  if (lane == 0) wcnt[parity][w] = __popcll(bal);
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    const int c = wcnt[parity][k];
+    off += k < w ? c : 0;
+    tot += c;
+  }
+  total = tot;
+  return off + below;
+}
+
+// per-atom fixed-point totals from the slot accumulators
+__device__ __forceinline__ void scatter_add(long long *p, double v) { fix_add(p, v); }
+__device__ __forceinline__ void scatter_add(long long *p, long long q) {
+  if (q != 0) atomicAdd((unsigned long long *)p, (unsigned long long)q);
+}
 
 // exclusive scan of vals[0..n) into out[0..n], out[n] = total (one wave)
 __device__ __forceinline__ int wave_exclusive_scan(const int *vals, int *out, int n) {
@@ -421,22 +468,25 @@ __device__ __forceinline__ int wave_exclusive_scan(const int *vals, int *out, in
 
 // children of order N: pairs (u, later sibling v) of the nodes of order N-1
 // in buffer `cb` (n nodes), kept ones into buffer 1-cb; returns their count
-// (LC + 1 on overflow)
-template <int N, int CAP, bool PB, int LC, bool STORE>
-__device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, double (&root)[TG_N], float df,
+// (LC + 1 on overflow).  W waves share the candidates; wave 0 runs the scans.
+template <int N, int CAP, bool PB, int LC, int W, bool STORE>
+__device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC, W> &S, int cb, int n, double (&root)[TG_N], float df,
                                          unsigned &nkept, unsigned &neval) {
-  const int lane = threadIdx.x & 63;
+  constexpr int NT = 64 * W;
+  const int tid = threadIdx.x;
   const int nb = 1 - cb;
-  for (int k = lane; k < n; k += 64) {
+  for (int k = tid; k < n; k += NT) {
     S.cnt[k] = S.gend[cb][k] - k - 1;     // later siblings of node k
   }
   __syncthreads();
-  const int P = wave_exclusive_scan(S.cnt, S.pref, n);
-  for (int k = lane; k < n; k += 64) S.cnt[k] = 0;
+  if (tid < 64) wave_exclusive_scan(S.cnt, S.pref, n);
   __syncthreads();
-  int nn = 0;
-  for (int base = 0; base < P; base += 64) {
-    const int q = base + lane;
+  const int P = S.pref[n];
+  for (int k = tid; k < n; k += NT) S.cnt[k] = 0;
+  __syncthreads();
+  int nn = 0, parity = 0;
+  for (int base = 0; base < P; base += NT, parity ^= 1) {
+    const int q = base + tid;
     bool keep = false;
     int uu = 0, sb = 0;
     OvlOut o;
@@ -455,7 +505,8 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, 
       keep = o.Vp > FLT_MIN;
     }
     neval += (unsigned)__popcll(__ballot(q < P));
-    const unsigned long long bal = __ballot(keep);
+    int total;
+    const int pos = nn + pass_compact<W>(S.wcnt, parity, __ballot(keep), total);
     if (keep) {
       const unsigned long long pu = S.path[cb][uu];
       int mem[N];
@@ -465,7 +516,6 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, 
       mem[N - 1] = sb;
       add_contrib<N, CAP, PB>(S, o, mem, root, df);
       if (STORE) {
-        const int pos = nn + __popcll(bal & ((1ull << lane) - 1ull));
         if (pos < LC) {
           S.ga[nb][pos] = o.a; S.gp[nb][pos] = o.p;
           S.gx[nb][pos] = o.cx; S.gy[nb][pos] = o.cy; S.gz[nb][pos] = o.cz;
@@ -475,16 +525,16 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, 
         atomicAdd(&S.cnt[uu], 1);
       }
     }
-    nn += __popcll(bal);
+    nn += total;
   }
-  nkept += (unsigned)nn;
+  nkept += (unsigned)nn;                // block-uniform
   __syncthreads();
   if (!STORE) return 0;
   if (nn > LC) return LC + 1;
   // a node's siblings are the kept children of its parent, contiguous
-  wave_exclusive_scan(S.cnt, S.pref, n);
+  if (tid < 64) wave_exclusive_scan(S.cnt, S.pref, n);
   __syncthreads();
-  for (int k = lane; k < nn; k += 64) {
+  for (int k = tid; k < nn; k += NT) {
     const int par = S.gpar[nb][k];
     S.gend[nb][k] = S.pref[par] + S.cnt[par];
   }
@@ -492,15 +542,18 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC> &S, int cb, int n, 
   return nn;
 }
 
-// one root of the breadth-first tree (the whole wave); results scattered to
-// the atoms, or the root queued for the depth-first kernel on overflow
-template <int CAP, bool PB, int LC>
-__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int list, unsigned long long &tk,
+// one root of the breadth-first tree (the whole block); results scattered to
+// the atoms, or the root queued for the next tier on overflow.  With W > 1
+// the block's waves split every order's candidates and the per-slot sums are
+// fixed point, so the result does not depend on the waves' timing.
+template <int CAP, bool PB, int LC, int W>
+__device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, int list, unsigned long long &tk,
                               unsigned long long &te) {
-  const int lane = threadIdx.x;
+  constexpr int NT = 64 * W;
+  const int tid = threadIdx.x, lane = tid & 63;
   const int i = d.hrow[h];
   const int m = d.nne[h];
-  for (int s = lane; s <= m; s += 64) {
+  for (int s = tid; s <= m; s += NT) {
     const int at = s == 0 ? i : d.nbr[(size_t)h * d.cap + s - 1];
     const float4 p = d.pos[at];
     S.idx[s] = at;
@@ -511,7 +564,7 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int l
       S.wc[s] = d.gammap_f[at]; S.wv[s] = d.dervs_f[at]; S.wg[s] = d.derus_f[at];
     }
 #pragma unroll
-    for (int v = 0; v < (PB ? TG_N : 2); v++) S.acc[v][s] = 0.0;
+    for (int v = 0; v < (PB ? TG_N : 2); v++) S.acc[v][s] = 0;
   }
   __syncthreads();
   double root[TG_N];
@@ -524,8 +577,9 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int l
   int n = 0;
   {
     const Gauss rg = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
-    for (int base = 0; base < m; base += 64) {
-      const int c = base + lane;
+    int parity = 0;
+    for (int base = 0; base < m; base += NT, parity ^= 1) {
+      const int c = base + tid;
       bool keep = false;
       int sb = 0;
       OvlOut o;
@@ -535,35 +589,35 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int l
         keep = o.Vp > FLT_MIN;
       }
       neval += (unsigned)__popcll(__ballot(c < m));
-      const unsigned long long bal = __ballot(keep);
+      int total;
+      const int pos = n + pass_compact<W>(S.wcnt, parity, __ballot(keep), total);
       if (keep) {
         const int mem[2] = {0, sb};
         add_contrib<2, CAP, PB>(S, o, mem, root, df);
-        const int pos = n + __popcll(bal & ((1ull << lane) - 1ull));
         if (pos < LC) {
           S.ga[0][pos] = o.a; S.gp[0][pos] = o.p;
           S.gx[0][pos] = o.cx; S.gy[0][pos] = o.cy; S.gz[0][pos] = o.cz;
           S.path[0][pos] = (unsigned long long)sb;
         }
       }
-      n += __popcll(bal);
+      n += total;
     }
     nkept += (unsigned)n;
     if (n > LC) ovf = true;
-    for (int k = lane; k < n && k < LC; k += 64) S.gend[0][k] = n;
+    for (int k = tid; k < n && k < LC; k += NT) S.gend[0][k] = n;
     __syncthreads();
   }
   if (!ovf && n > 1) {
-    n = bfs_order<3, CAP, PB, LC, true>(S, 0, n, root, df, nkept, neval);
+    n = bfs_order<3, CAP, PB, LC, W, true>(S, 0, n, root, df, nkept, neval);
     if (n > LC) ovf = true;
     else if (n > 1) {
-      n = bfs_order<4, CAP, PB, LC, true>(S, 1, n, root, df, nkept, neval);
+      n = bfs_order<4, CAP, PB, LC, W, true>(S, 1, n, root, df, nkept, neval);
       if (n > LC) ovf = true;
-      else if (n > 1) bfs_order<5, CAP, PB, LC, false>(S, 0, n, root, df, nkept, neval);
+      else if (n > 1) bfs_order<5, CAP, PB, LC, W, false>(S, 0, n, root, df, nkept, neval);
     }
   }
-  if (ovf) {                              // wave-uniform: the next tier takes this root
-    if (lane == 0) {
+  if (ovf) {                              // block-uniform: the next tier takes this root
+    if (tid == 0) {
       const int q = 4 + 2 * list + (PB ? 1 : 0);
       const unsigned k = atomicAdd(&d.counters[q], 1ull);
       d.tree_ovf[(size_t)(q - 4) * d.nh + k] = h;
@@ -574,30 +628,31 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC> &S, int h, int l
 #pragma unroll
   for (int v = 0; v < (PB ? TG_N : 2); v++) {
     const double r = wave_sum(root[v]);
-    if (lane == 0) S.acc[v][0] += r;
+    if (lane == 0) slot_add(&S.acc[v][0], r);
   }
   __syncthreads();
-  for (int s = lane; s <= m; s += 64) {
+  for (int s = tid; s <= m; s += NT) {
     const int at = S.idx[s];
     if (!PB) {
-      fix_add(&d.acc_vol[at], S.acc[0][s]);
-      fix_add(&d.acc_sa[at], S.acc[1][s]);
+      scatter_add(&d.acc_vol[at], S.acc[0][s]);
+      scatter_add(&d.acc_sa[at], S.acc[1][s]);
     } else {
 #pragma unroll
-      for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], S.acc[v][s]);
+      for (int v = 0; v < TG_N; v++) scatter_add(&d.acc_grad[(size_t)at * TG_N + v], S.acc[v][s]);
     }
   }
-  tk += nkept;                          // wave-uniform; added to the counters once per block
-  te += neval;
+  tk += nkept;                          // block-uniform: wave 0 reports it
+  te += neval;                          // this wave's evaluations
+  if (W > 1) __syncthreads();           // the next root restages the slots
 }
 
-// overlaps kept and evaluated by one persistent block (pass A only)
+// overlaps kept (block totals) and evaluated (per wave) by one persistent
+// block (pass A only)
 __device__ __forceinline__ void add_tree_counts(const Dev &d, bool pass_b, unsigned long long tk,
                                                 unsigned long long te) {
-  if (!pass_b && threadIdx.x == 0 && (tk | te)) {
-    atomicAdd(&d.counters[0], tk);
-    atomicAdd(&d.counters[1], te);
-  }
+  if (pass_b || (threadIdx.x & 63) != 0) return;
+  if (threadIdx.x == 0 && tk) atomicAdd(&d.counters[0], tk);
+  if (te) atomicAdd(&d.counters[1], te);
 }
 
 
@@ -605,7 +660,7 @@ __device__ __forceinline__ void add_tree_counts(const Dev &d, bool pass_b, unsig
 // overflowing roots go to the second tier's list
 template <int CAP, bool PB>
 __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
-  __shared__ BfsLds<CAP, PB, kLevelCap> S;
+  __shared__ BfsLds<CAP, PB, kLevelCap, 1> S;
   __shared__ int slot;
   unsigned *head = &d.queue[PB ? Q_TREE_B : Q_TREE_A];
   // roots in runs of `run` per queue grab when there are many per block
@@ -613,27 +668,30 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   const int run = max(1, min(8, d.nh / (int)(gridDim.x * 8)));
   unsigned long long tk = 0, te = 0;
   for (int h0 = next_item_n(head, &slot, run); h0 < d.nh; h0 = next_item_n(head, &slot, run))
-    for (int h = h0; h < min(h0 + run, d.nh); h++) tree_bfs_root<CAP, PB, kLevelCap>(d, S, h, 0, tk, te);
+    for (int h = h0; h < min(h0 + run, d.nh); h++) tree_bfs_root<CAP, PB, kLevelCap, 1>(d, S, h, 0, tk, te);
   add_tree_counts(d, PB, tk, te);
 }
 
 // second tier: the roots the first tier queued, with 1536-node levels in
-// dynamic LDS; what still overflows goes to the depth-first kernel
+// dynamic LDS (one block per CU) shared by kBigWaves waves; what still
+// overflows goes to the depth-first kernel
+constexpr int kBigWaves = 4;
 template <int CAP, bool PB>
-__global__ __launch_bounds__(64) void k_tree_bfs_big(Dev d) {
+__global__ __launch_bounds__(64 * kBigWaves) void k_tree_bfs_big(Dev d) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  BfsLds<CAP, PB, kLevelCapBig> &S = *reinterpret_cast<BfsLds<CAP, PB, kLevelCapBig> *>(lds_raw);
+  using L = BfsLds<CAP, PB, kLevelCapBig, kBigWaves>;
+  L &S = *reinterpret_cast<L *>(lds_raw);
   __shared__ int slot;
   const int count = (int)d.counters[4 + (PB ? 1 : 0)];
   unsigned *head = &d.queue[PB ? Q_TREE_BB : Q_TREE_BA];
   unsigned long long tk = 0, te = 0;
   for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
-    tree_bfs_root<CAP, PB, kLevelCapBig>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1, tk, te);
+    tree_bfs_root<CAP, PB, kLevelCapBig, kBigWaves>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1, tk, te);
   add_tree_counts(d, PB, tk, te);
 }
 
 template <int CAP, bool PB>
-static size_t big_lds() { return sizeof(BfsLds<CAP, PB, kLevelCapBig>); }
+static size_t big_lds() { return sizeof(BfsLds<CAP, PB, kLevelCapBig, kBigWaves>); }
 
 hipError_t prepare_tree() {
   hipError_t e = hipSuccess;
@@ -657,11 +715,11 @@ static void launch_tree_cap(const Dev &d, int pass, hipStream_t s) {
   const size_t la = big_lds<CAP, false>(), lb = big_lds<CAP, true>();
   if (pass == 0) {
     hipLaunchKernelGGL((k_tree_bfs<CAP, false>), dim3(g), dim3(64), 0, s, d);
-    hipLaunchKernelGGL((k_tree_bfs_big<CAP, false>), dim3(gf), dim3(64), la, s, d);
+    hipLaunchKernelGGL((k_tree_bfs_big<CAP, false>), dim3(gf), dim3(64 * kBigWaves), la, s, d);
     hipLaunchKernelGGL((k_tree<CAP, false>), dim3(gf), dim3(64), 0, s, d);
   } else {
     hipLaunchKernelGGL((k_tree_bfs<CAP, true>), dim3(g), dim3(64), 0, s, d);
-    hipLaunchKernelGGL((k_tree_bfs_big<CAP, true>), dim3(gf), dim3(64), lb, s, d);
+    hipLaunchKernelGGL((k_tree_bfs_big<CAP, true>), dim3(gf), dim3(64 * kBigWaves), lb, s, d);
     hipLaunchKernelGGL((k_tree<CAP, true>), dim3(gf), dim3(64), 0, s, d);
   }
 }
