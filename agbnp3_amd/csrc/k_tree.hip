@@ -89,13 +89,73 @@ __device__ __forceinline__ float sa_term(const OvlOut &o, float sign, float a, f
   return sign * o.fp * a * o.V * (3.f * o.dinv + 2.f * t) * __builtin_amdgcn_rcpf(R);
 }
 
-// add to a per-slot accumulator: FP64 LDS atomics (one wave per root: the
-// lanes' order is fixed), or 64-bit fixed point with integer atomics (several
-// waves per root: the sum does not depend on their order)
+// add to a per-slot accumulator: FP64 LDS atomics with one wave per root
+// (the lanes' order is fixed), or 64-bit fixed point with integer atomics
+// with several waves per root (the sum does not depend on their order)
 __device__ __forceinline__ void slot_add(double *p, double v) { atomicAdd(p, v); }
 __device__ __forceinline__ void slot_add(long long *p, double v) {
   const long long q = __double2ll_rn(v * kFix);
   if (q != 0) atomicAdd((unsigned long long *)p, (unsigned long long)q);
+}
+
+// Pass-B terms of one kept overlap of order N for each member k: the
+// derivatives of its self-volume and surface contributions with respect to
+// x_k (agbnp3_der_vp_rooti, kernels.c:1601-1956, through U, V, H;
+// agbnp3_cavity_dersgb_rooti, kernels.c:1959-2405, through gamma', dervs,
+// derus).  With g_k = dV/dx_k = -2 a_k V (x_k - c), the radius term
+// G_k = a_k V (3/a + 2 |x_k - c|^2) / R_k, S1 = sum_k W_k G_k and
+// S2 = sum_k W_k g_k / R_k per weight set W (sign folded in), member k gets
+//   g_k (h3 S1 - 2 fp W_k / R_k [+ U or V coefficient]) + 2 fp a_k S2 / a
+// for the GB, vdW and cavity terms and H g_k for the HB term; h3 = f'' +
+// f'/V.  emit(k, out[TG_N]) receives them in member order.
+template <int N, typename Emit>
+__device__ __forceinline__ void member_derivs(const OvlOut &o, float cn, float df, const float (&A_)[N],
+                                              const float (&X_)[N], const float (&Y_)[N], const float (&Z_)[N],
+                                              const float (&R_)[N], const float (&WC)[N], const float (&WV)[N],
+                                              const float (&WG)[N], float Us, float Vs, float Hs, Emit emit) {
+  const float fourpi1 = 1.0f / (4.0f * kPiF);
+  const float Uc = -df * fourpi1 * cn * Us * o.fp;
+  const float Vc = -fourpi1 * cn * Vs * o.fp;
+  const float Hc = -cn * Hs * o.fp;
+  const float h3 = o.fpp + o.fp * __builtin_amdgcn_rcpf(o.V), h4 = 2.f * o.fp * o.dinv;
+  const float c3 = 3.f * o.dinv, m2V = -2.f * o.V, m2fp = -2.f * o.fp;
+  float g[N][3], iR[N];
+  float S1c = 0.f, S1v = 0.f, S1g = 0.f;
+  float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const float ex = X_[k] - o.cx, ey = Y_[k] - o.cy, ez = Z_[k] - o.cz;
+    const float av = A_[k] * o.V, mav = A_[k] * m2V;
+    iR[k] = __builtin_amdgcn_rcpf(R_[k]);
+    g[k][0] = mav * ex;
+    g[k][1] = mav * ey;
+    g[k][2] = mav * ez;
+    const float GR = av * (c3 + 2.f * (ex * ex + ey * ey + ez * ez)) * iR[k];
+    S1c += WC[k] * GR; S1v += WV[k] * GR; S1g += WG[k] * GR;
+    const float wc = WC[k] * iR[k], wv = WV[k] * iR[k], wg = WG[k] * iR[k];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      S2c[q] += wc * g[k][q];
+      S2v[q] += wv * g[k][q];
+      S2g[q] += wg * g[k][q];
+    }
+  }
+  const float Kc = h3 * S1c, Kv = h3 * S1v + Vc, Kg = h3 * S1g + Uc;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const float t = m2fp * iR[k], ha = h4 * A_[k];
+    const float kc = Kc + WC[k] * t, kv = Kv + WV[k] * t, kg = Kg + WG[k] * t;
+    float out[TG_N];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const float gq = g[k][q];
+      out[TG_GB + q] = gq * kg + ha * S2g[q];
+      out[TG_VW + q] = gq * kv + ha * S2v[q];
+      out[TG_CAV + q] = gq * kc + ha * S2c[q];
+      out[TG_HB + q] = Hc * gq;
+    }
+    emit(k, out);
+  }
 }
 
 // Contributions of one kept overlap o of order N with members mem[0..N-1]
@@ -125,54 +185,15 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
       WC[k] = sign * S.wc[sk]; WV[k] = sign * S.wv[sk]; WG[k] = sign * S.wg[sk];
       Us += S.u[sk]; Vs += S.v[sk]; Hs += S.h[sk];
     }
-    const float fourpi1 = 1.0f / (4.0f * kPiF);
-    const float Uc = -df * fourpi1 * cn * Us * o.fp;
-    const float Vc = -fourpi1 * cn * Vs * o.fp;
-    const float Hc = -cn * Hs * o.fp;
-    const float h3 = o.fpp + o.fp * __builtin_amdgcn_rcpf(o.V), h4 = 2.f * o.fp * o.dinv;
-    float gdr[N][3], gdR[N];
-    float S1c = 0.f, S1v = 0.f, S1g = 0.f;
-    float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      const float ex = X_[k] - o.cx, ey = Y_[k] - o.cy, ez = Z_[k] - o.cz;
-      const float av = A_[k] * o.V;
-      gdr[k][0] = -2.f * av * ex;
-      gdr[k][1] = -2.f * av * ey;
-      gdr[k][2] = -2.f * av * ez;
-      gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) * __builtin_amdgcn_rcpf(R_[k]);
-      S1c += WC[k] * gdR[k]; S1v += WV[k] * gdR[k]; S1g += WG[k] * gdR[k];
-      const float iR = __builtin_amdgcn_rcpf(R_[k]);
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        S2c[q] += WC[k] * gdr[k][q] * iR;
-        S2v[q] += WV[k] * gdr[k][q] * iR;
-        S2g[q] += WG[k] * gdr[k][q] * iR;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-      const float iR = __builtin_amdgcn_rcpf(R_[k]);
-      const float t = -2.f * o.fp * (1.f - A_[k] * o.dinv) * iR;
-      const float bc = h3 * S1c + WC[k] * t, bv = h3 * S1v + WV[k] * t, bg = h3 * S1g + WG[k] * t;
-      const float ha = h4 * A_[k];
-      float out[TG_N];
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const float g = gdr[k][q];
-        out[TG_GB + q] = g * bg + ha * (S2g[q] - WG[k] * g * iR) + Uc * g;
-        out[TG_VW + q] = g * bv + ha * (S2v[q] - WV[k] * g * iR) + Vc * g;
-        out[TG_CAV + q] = g * bc + ha * (S2c[q] - WC[k] * g * iR);
-        out[TG_HB + q] = Hc * g;
-      }
+    member_derivs<N>(o, cn, df, A_, X_, Y_, Z_, R_, WC, WV, WG, Us, Vs, Hs, [&](int k, const float (&out)[TG_N]) {
       if (k == 0) {
 #pragma unroll
         for (int v = 0; v < TG_N; v++) root[v] += (double)out[v];
       } else {
 #pragma unroll
-        for (int v = 0; v < TG_N; v++) slot_add(&S.acc[v][mem[k]], (double)out[v]);
+        for (int v = 0; v < TG_N; v++) slot_add(&S.acc[v][mem[k]], out[v]);
       }
-    }
+    });
   }
 }
 
@@ -242,46 +263,8 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
         A_[N - 1] = ab; X_[N - 1] = xb; Y_[N - 1] = yb; Z_[N - 1] = zb; R_[N - 1] = Rb;
         WC[N - 1] = sign * S.wc[sb]; WV[N - 1] = sign * S.wv[sb]; WG[N - 1] = sign * S.wg[sb];
         const float Us = pU + S.u[sb], Vs = pV + S.v[sb], Hs = pH + S.h[sb];
-        const float fourpi1 = 1.0f / (4.0f * kPiF);
-        const float Uc = -df * fourpi1 * cn * Us * o.fp;
-        const float Vc = -fourpi1 * cn * Vs * o.fp;
-        const float Hc = -cn * Hs * o.fp;
-        const float h3 = o.fpp + o.fp * __builtin_amdgcn_rcpf(o.V), h4 = 2.f * o.fp * o.dinv;
-        float gdr[N][3], gdR[N];
-        float S1c = 0.f, S1v = 0.f, S1g = 0.f;
-        float S2c[3] = {0.f, 0.f, 0.f}, S2v[3] = {0.f, 0.f, 0.f}, S2g[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-          const float ex = X_[k] - o.cx, ey = Y_[k] - o.cy, ez = Z_[k] - o.cz;
-          const float av = A_[k] * o.V;
-          gdr[k][0] = -2.f * av * ex;
-          gdr[k][1] = -2.f * av * ey;
-          gdr[k][2] = -2.f * av * ez;
-          gdR[k] = av * (3.f * o.dinv + 2.f * (ex * ex + ey * ey + ez * ez)) * __builtin_amdgcn_rcpf(R_[k]);
-          S1c += WC[k] * gdR[k]; S1v += WV[k] * gdR[k]; S1g += WG[k] * gdR[k];
-          const float iR = __builtin_amdgcn_rcpf(R_[k]);
-#pragma unroll
-          for (int q = 0; q < 3; q++) {
-            S2c[q] += WC[k] * gdr[k][q] * iR;
-            S2v[q] += WV[k] * gdr[k][q] * iR;
-            S2g[q] += WG[k] * gdr[k][q] * iR;
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-          const float iR = __builtin_amdgcn_rcpf(R_[k]);
-          const float t = -2.f * o.fp * (1.f - A_[k] * o.dinv) * iR;
-          const float bc = h3 * S1c + WC[k] * t, bv = h3 * S1v + WV[k] * t, bg = h3 * S1g + WG[k] * t;
-          const float ha = h4 * A_[k];
-          float out[TG_N];
-#pragma unroll
-          for (int q = 0; q < 3; q++) {
-            const float g = gdr[k][q];
-            out[TG_GB + q] = g * bg + ha * (S2g[q] - WG[k] * g * iR) + Uc * g;
-            out[TG_VW + q] = g * bv + ha * (S2v[q] - WV[k] * g * iR) + Vc * g;
-            out[TG_CAV + q] = g * bc + ha * (S2c[q] - WC[k] * g * iR);
-            out[TG_HB + q] = Hc * g;
-          }
+        member_derivs<N>(o, cn, df, A_, X_, Y_, Z_, R_, WC, WV, WG, Us, Vs, Hs,
+                         [&](int k, const float (&out)[TG_N]) {
           if (k == 0) {
 #pragma unroll
             for (int v = 0; v < TG_N; v++) root[v] += (double)out[v];
@@ -290,7 +273,7 @@ __device__ __forceinline__ int eval_children(TreeLds<CAP, PB> &S, const Gauss &p
 #pragma unroll
             for (int v = 0; v < TG_N; v++) slot_add(&S.acc[v][slot], (double)out[v]);
           }
-        }
+        });
       }
       if (STORE) {
         const int pos = cnt + __popcll(bal & ((1ull << lane) - 1ull));
@@ -398,7 +381,8 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
 constexpr int kLevelCap = 128;          // nodes of one order held in LDS
 constexpr int kLevelCapBig = 1536;      // second tier (dynamic LDS) for the largest trees
 
-// slot accumulators: FP64 for one wave per root, fixed point for several
+// slot accumulators: FP64 with one wave per root (FP64 LDS atomics measured
+// faster than FP32 ones here), fixed point with several waves
 template <int W> struct SlotAcc { using T = long long; };
 template <> struct SlotAcc<1> { using T = double; };
 
@@ -625,13 +609,25 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, in
     __syncthreads();
     return;
   }
+  // the root's own sums (slot 0 takes no member adds): lane 0 of the wave,
+  // or with several waves fixed-point adds of each wave's sum
 #pragma unroll
   for (int v = 0; v < (PB ? TG_N : 2); v++) {
-    const double r = wave_sum(root[v]);
-    if (lane == 0) slot_add(&S.acc[v][0], r);
+    root[v] = wave_sum(root[v]);
+    if (W > 1 && lane == 0) slot_add(&S.acc[v][0], root[v]);
   }
-  __syncthreads();
-  for (int s = tid; s <= m; s += NT) {
+  if (W > 1) __syncthreads();
+  if (W == 1 && tid == 0) {
+    const int at = S.idx[0];
+    if (!PB) {
+      fix_add(&d.acc_vol[at], root[0]);
+      fix_add(&d.acc_sa[at], root[1]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < TG_N; v++) fix_add(&d.acc_grad[(size_t)at * TG_N + v], root[v]);
+    }
+  }
+  for (int s = (W == 1 ? 1 + tid : tid); s <= m; s += NT) {
     const int at = S.idx[s];
     if (!PB) {
       scatter_add(&d.acc_vol[at], S.acc[0][s]);
