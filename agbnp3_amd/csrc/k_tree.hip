@@ -168,22 +168,28 @@ __device__ __forceinline__ void add_contrib(Lds &S, const OvlOut &o, const int (
   if (!PB) {
     const double wv = (double)cn * (double)o.Vp;
     root[0] += wv;
-    root[1] += (double)sa_term(o, sign, S.a[0], S.x[0], S.y[0], S.z[0], S.R[0]);
+    const float4 p0 = S.p[0];
+    root[1] += (double)sa_term(o, sign, p0.w, p0.x, p0.y, p0.z, S.r[0]);
+    float sa[N];
 #pragma unroll
     for (int k = 1; k < N; k++) {
-      const int sk = mem[k];
-      slot_add(&S.acc[0][sk], wv);
-      slot_add(&S.acc[1][sk], (double)sa_term(o, sign, S.a[sk], S.x[sk], S.y[sk], S.z[sk], S.R[sk]));
+      const float4 pk = S.p[mem[k]];
+      sa[k] = sa_term(o, sign, pk.w, pk.x, pk.y, pk.z, S.r[mem[k]]);
+    }
+#pragma unroll
+    for (int k = 1; k < N; k++) {
+      slot_add(&S.acc[0][mem[k]], wv);
+      slot_add(&S.acc[1][mem[k]], (double)sa[k]);
     }
   } else {
     float A_[N], X_[N], Y_[N], Z_[N], R_[N], WC[N], WV[N], WG[N];
     float Us = 0.f, Vs = 0.f, Hs = 0.f;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-      const int sk = mem[k];
-      A_[k] = S.a[sk]; X_[k] = S.x[sk]; Y_[k] = S.y[sk]; Z_[k] = S.z[sk]; R_[k] = S.R[sk];
-      WC[k] = sign * S.wc[sk]; WV[k] = sign * S.wv[sk]; WG[k] = sign * S.wg[sk];
-      Us += S.u[sk]; Vs += S.v[sk]; Hs += S.h[sk];
+      const float4 pk = S.p[mem[k]], qk = S.q[mem[k]], wk = S.w[mem[k]];
+      A_[k] = pk.w; X_[k] = pk.x; Y_[k] = pk.y; Z_[k] = pk.z; R_[k] = qk.x;
+      WC[k] = sign * wk.x; WV[k] = sign * wk.y; WG[k] = sign * wk.z;
+      Us += qk.y; Vs += qk.z; Hs += qk.w;
     }
     member_derivs<N>(o, cn, df, A_, X_, Y_, Z_, R_, WC, WV, WG, Us, Vs, Hs, [&](int k, const float (&out)[TG_N]) {
       if (k == 0) {
@@ -389,9 +395,11 @@ template <> struct SlotAcc<1> { using T = double; };
 template <int CAP, bool PB, int LC, int W>
 struct BfsLds {
   int idx[CAP + 1];
-  float a[CAP + 1], x[CAP + 1], y[CAP + 1], z[CAP + 1], R[CAP + 1];
-  float u[PB ? CAP + 1 : 1], v[PB ? CAP + 1 : 1], h[PB ? CAP + 1 : 1];
-  float wc[PB ? CAP + 1 : 1], wv[PB ? CAP + 1 : 1], wg[PB ? CAP + 1 : 1];
+  // slot atoms, one 16-byte read per record: x, y, z, Gaussian exponent;
+  // pass A radius; pass B (radius, deru, derv, derh), (gamma', dervs, derus)
+  float4 p[CAP + 1];
+  float r[PB ? 1 : CAP + 1];
+  float4 q[PB ? CAP + 1 : 1], w[PB ? CAP + 1 : 1];
   typename SlotAcc<W>::T acc[PB ? TG_N : 2][CAP + 1];
   // two node buffers: Gaussian, member slots (16 bits each, root excluded),
   // end of the node's sibling group, parent index in the previous order
@@ -485,7 +493,8 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC, W> &S, int cb, int 
       const int vv = uu + 1 + (q - S.pref[uu]);
       sb = (int)((S.path[cb][vv] >> (16 * (N - 3))) & 0xFFFFull);    // last atom of sibling v
       const Gauss par = {S.ga[cb][uu], S.gp[cb][uu], S.gx[cb][uu], S.gy[cb][uu], S.gz[cb][uu]};
-      o = overlap(par, S.a[sb], S.x[sb], S.y[sb], S.z[sb]);
+      const float4 pb = S.p[sb];
+      o = overlap(par, pb.w, pb.x, pb.y, pb.z);
       keep = o.Vp > FLT_MIN;
     }
     neval += (unsigned)__popcll(__ballot(q < P));
@@ -541,11 +550,12 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, in
     const int at = s == 0 ? i : d.nbr[(size_t)h * d.cap + s - 1];
     const float4 p = d.pos[at];
     S.idx[s] = at;
-    S.a[s] = d.galpha[at];
-    S.x[s] = p.x; S.y[s] = p.y; S.z[s] = p.z; S.R[s] = p.w;
+    S.p[s] = make_float4(p.x, p.y, p.z, d.galpha[at]);
     if (PB) {
-      S.u[s] = d.deru_f[at]; S.v[s] = d.derv_f[at]; S.h[s] = d.derh_f[at];
-      S.wc[s] = d.gammap_f[at]; S.wv[s] = d.dervs_f[at]; S.wg[s] = d.derus_f[at];
+      S.q[s] = make_float4(p.w, d.deru_f[at], d.derv_f[at], d.derh_f[at]);
+      S.w[s] = make_float4(d.gammap_f[at], d.dervs_f[at], d.derus_f[at], 0.f);
+    } else {
+      S.r[s] = p.w;
     }
 #pragma unroll
     for (int v = 0; v < (PB ? TG_N : 2); v++) S.acc[v][s] = 0;
@@ -560,7 +570,8 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, in
   // order 2: the root with every list entry, one sibling group
   int n = 0;
   {
-    const Gauss rg = {S.a[0], kPFC, S.x[0], S.y[0], S.z[0]};
+    const float4 p0 = S.p[0];
+    const Gauss rg = {p0.w, kPFC, p0.x, p0.y, p0.z};
     int parity = 0;
     for (int base = 0; base < m; base += NT, parity ^= 1) {
       const int c = base + tid;
@@ -569,7 +580,8 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, in
       OvlOut o;
       if (c < m) {
         sb = c + 1;
-        o = overlap(rg, S.a[sb], S.x[sb], S.y[sb], S.z[sb]);
+        const float4 pb = S.p[sb];
+        o = overlap(rg, pb.w, pb.x, pb.y, pb.z);
         keep = o.Vp > FLT_MIN;
       }
       neval += (unsigned)__popcll(__ballot(c < m));
