@@ -106,11 +106,13 @@ void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s) {
 //   e   = q_i q_j / f,  f = sqrt(d2 + B_i B_j exp(-d2 / 4 B_i B_j))
 //   g_i += q_i q_j (1 - e/4) / f^3 (x_j - x_i),  g_j -= same      (x 2 df later)
 //   A_i, A_j += q_i q_j (B_i B_j + d2/4) exp(.) / f^3
-// An item is a tile pair of up to 256 x 256 atoms of one system.  Each wave
-// holds 64 row atoms in its lanes and sweeps the column atoms 64 at a time:
-// the column atom, its data and its partial sums move one lane per step with
-// a DPP wave rotation, so every lane meets every column once without LDS
-// traffic.  Diagonal tiles see each pair twice and keep only the row side.
+// An item is a tile pair of up to 256 x 256 atoms of one system, its columns
+// staged in LDS.  Each wave holds 64 row atoms in its lanes and sweeps the
+// columns 64 at a time: at step s lane l reads column (l - s) mod 64 of the
+// chunk, and the column's partial sums move one lane per step with a DPP
+// wave rotation, so every lane meets every column once and the sums end
+// where they started.  Diagonal tiles see each pair twice and keep only the
+// row side.
 __device__ __forceinline__ float wave_rot(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, true));
 }
@@ -118,6 +120,8 @@ __device__ __forceinline__ int wave_rot(int v) { return __builtin_amdgcn_update_
 
 __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
   __shared__ float colacc[4][4][256];   // [wave][A, gx, gy, gz][column]: fixed-order sum
+  __shared__ float4 cxq[256];           // tile columns: x, y, z, charge
+  __shared__ float2 cbr[256];           // Born radius and its reciprocal
   __shared__ double ered[4];
   __shared__ int slot;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -127,6 +131,19 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
     const int i = tp.x + threadIdx.x;
     const bool wave_rows = tp.x + w * 64 < tp.y;   // wave-uniform: any valid row
     const int nch = (tp.w - tp.z + 63) >> 6;
+    {
+      const int j = tp.z + threadIdx.x;      // padding columns: far, no charge
+      float4 c = make_float4(-1e6f - (float)threadIdx.x, 0.f, 0.f, 0.f);
+      float2 b = make_float2(1.f, 1.f);
+      if (j < tp.w) {
+        const float4 p = d.pos[j];
+        const float bj = d.br_f[j];
+        c = make_float4(p.x, p.y, p.z, d.q[j]);
+        b = make_float2(bj, __builtin_amdgcn_rcpf(bj));
+      }
+      cxq[threadIdx.x] = c;
+      cbr[threadIdx.x] = b;
+    }
     float xi = 1e6f + (float)threadIdx.x, yi = 0.f, zi = 0.f, qi = 0.f, bi = 1.f;   // padding: far, no charge
     if (i < tp.y) {
       const float4 p = d.pos[i];
@@ -134,25 +151,23 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
       qi = d.q[i];
       bi = d.br_f[i];
     }
+    const float qib = -0.25f * __builtin_amdgcn_rcpf(bi);   // exp(-d2 / (4 B_i B_j)) = exp(d2 qib / B_j)
+    __syncthreads();
     float rA = 0.f, rx = 0.f, ry = 0.f, rz = 0.f, re = 0.f;
     for (int c = 0; c < nch; c++) {
       float cA = 0.f, cgx = 0.f, cgy = 0.f, cgz = 0.f;
       if (wave_rows) {
-        const int j0 = tp.z + c * 64 + lane;
-        float cx = -1e6f - (float)lane, cy = 0.f, cz = 0.f, cq = 0.f, cb = 1.f;
-        if (j0 < tp.w) {
-          const float4 p = d.pos[j0];
-          cx = p.x; cy = p.y; cz = p.z;
-          cq = d.q[j0];
-          cb = d.br_f[j0];
-        }
+        // the column whose partial sums this lane holds at step s
         for (int s = 0; s < 64; s++) {
-          const float ex = cx - xi, ey = cy - yi, ez = cz - zi;
+          const int col = c * 64 + ((lane - s) & 63);
+          const float4 cp = cxq[col];
+          const float2 cb = cbr[col];
+          const float ex = cp.x - xi, ey = cp.y - yi, ez = cp.z - zi;
           const float d2 = ex * ex + ey * ey + ez * ez;
           // the reference's mask (kernels.c:107): d2 > 0, which also drops the self pair
-          const float qq = d2 > 0.f ? qi * cq : 0.f;
-          const float bb = bi * cb;
-          const float et = __expf(-0.25f * d2 * __builtin_amdgcn_rcpf(bb));
+          const float qq = d2 > 0.f ? qi * cp.w : 0.f;
+          const float bb = bi * cb.x;
+          const float et = __expf(d2 * (qib * cb.y));
           const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
           const float fgb3 = fgb * fgb * fgb;
           re += qq * fgb;
@@ -162,7 +177,6 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
           rA += at;
           cA += at;
           cgx -= mw * ex; cgy -= mw * ey; cgz -= mw * ez;
-          cx = wave_rot(cx); cy = wave_rot(cy); cz = wave_rot(cz); cq = wave_rot(cq); cb = wave_rot(cb);
           cA = wave_rot(cA); cgx = wave_rot(cgx); cgy = wave_rot(cgy); cgz = wave_rot(cgz);
         }
         // 64 rotations bring every column's sums back to its starting lane
