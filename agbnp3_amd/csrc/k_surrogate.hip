@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_sg_features(Dev d) {
       const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const float d2 = ex * ex + ey * ey + ez * ez;
       if (!(d2 < kSgCut * kSgCut)) continue;
-      const float dist = d2 * rsqrtf(d2);
+      const float dist = d2 * __builtin_amdgcn_rsqf(d2);
       float T, Tp, b[4], db[4];
       sg_taper(dist, T, Tp);
       T *= d.sg_sgam[u] * kSgBinScale;
@@ -720,7 +720,7 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
       const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;   // x_a - x_j
       const float d2 = ex * ex + ey * ey + ez * ez;
       if (!(d2 < kSgCut * kSgCut)) continue;
-      const float inv = rsqrtf(d2), dist = d2 * inv;
+      const float inv = __builtin_amdgcn_rsqf(d2), dist = d2 * inv;
       float T, Tp, b[4], db[4];
       sg_taper(dist, T, Tp);
       const int ks = sg_basis(dist, b, db);
