@@ -97,6 +97,7 @@ struct SysInfo {
 
 // ---- Born-radius surrogate (MLP 66 -> 128 -> 128 -> 128 -> 1)
 constexpr int kSgNShell = 16, kSgNFeat = 66, kSgKPad = 96, kSgHid = 128;
+constexpr int kSgPad = 4, kSgRow = 24;   // padded ds/dF rows: zeros | 16 shells | zeros
 constexpr int kSgBlk = 64;               // atoms per descriptor row block
 constexpr int kSgNbrCap = 1536;          // neighbours per atom within kSgCut + kSgSkin
 constexpr float kSgCut = 12.0f, kSgSkin = 1.5f;
@@ -235,18 +236,20 @@ struct Dev {
   float4 *sg_ref;                      // [n] positions the lists were built from
   int *sg_slot;                        // [n] slot of each atom in sg_catoms
   // per call, in slot order: position and R, self-volume fraction (0 for
-  // hydrogens), and (q2ab, abrw) for the gradient pass; sg_F, sg_X, sg_s and
-  // sg_dsdF are also in slot order
+  // hydrogens), and (q2ab, abrw, self-volume fraction, 0) for the gradient
+  // pass (one gather per neighbour); sg_F, sg_X, sg_s and sg_dsdF are also in
+  // slot order
   float4 *sg_spos;
   float *sg_sgam;
-  float2 *sg_sw;
+  float4 *sg_sw;
   int *sg_nbl;                         // per system [0] built, [1] rebuild this call, [2] builds, [3] overflow
   GridParams *sg_grid_p;               // per system all-atom cell grid
   int *sg_cstart, *sg_catoms, *sg_ctmp;  // [nsys][kMaxCells + 1], [n], [n]
   float *sg_F;                         // raw shell descriptors [n][64] (slot order)
   float *sg_X;                         // normalised input [n][kSgKPad] (slot order)
   float *sg_s;                         // predicted s [n] (slot order)
-  float *sg_dsdF;                      // ds/dF by channel: [4][n][16] (slot order)
+  float *sg_dsdF;                      // ds/dF by channel: [4][n][kSgRow] (slot order): kSgPad zeros, the
+                                       // 16 shells, zeros (any 4 consecutive shells of a B-spline one read)
   float *sg_target;                    // plain descreening sum of the call [n]
   double *sg_c, *sg_delta;             // anchor corrections, surrogate error at the anchor [n]
   double *sg_err;                      // per system: first-order Delta G error estimate

@@ -1763,7 +1763,7 @@ static int pack_sg_attach(Pack &P, int model, int every) {
     const size_t ns = std::max(P.nsys, 1);
     if (upload(L, &d.sys_sg, flags) || upload(L, &d.sg_rows, rows) || dalloc(L, &d.sg_state, 4 * ns) ||
         dalloc(L, &d.sg_any_plain, 1) || dalloc(L, &d.sg_F, 64 * nn) || dalloc(L, &d.sg_X, (size_t)kSgKPad * nn) ||
-        dalloc(L, &d.sg_s, nn) || dalloc(L, &d.sg_dsdF, 64 * nn) || dalloc(L, &d.sg_target, nn) ||
+        dalloc(L, &d.sg_s, nn) || dalloc(L, &d.sg_dsdF, 4 * kSgRow * nn) || dalloc(L, &d.sg_target, nn) ||
         dalloc(L, &d.sg_c, nn) || dalloc(L, &d.sg_delta, nn) || dalloc(L, &d.sg_err, ns) ||
         dalloc(L, &d.sg_nbr, rows.size() * kSgNbrCap * kSgBlk) || dalloc(L, &d.sg_nn, nn) ||
         dalloc(L, &d.sg_ref, nn) || dalloc(L, &d.sg_nbl, 4 * ns) || dalloc(L, &d.sg_grid_p, ns) ||
@@ -1775,6 +1775,8 @@ static int pack_sg_attach(Pack &P, int model, int every) {
     // change of model (the guard compares against the positions they were
     // built from)
     HIPCHK(hipMemset(d.sg_nbl, 0, sizeof(int) * 4 * ns));
+    // the pads of the ds/dF rows stay zero (calls write the 16 shells only)
+    HIPCHK(hipMemset(d.sg_dsdF, 0, sizeof(float) * 4 * kSgRow * nn));
   }
   HIPCHK(hipMemset(d.sg_state, 0, sizeof(int) * 4 * P.nsys));
   HIPCHK(hipMemset(d.sg_c, 0, sizeof(double) * std::max(P.n, 1)));

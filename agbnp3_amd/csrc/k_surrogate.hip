@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_sg_shadow_w(Dev d) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= d.n || !d.sys_sg[d.asys[t]]) return;
   const int a = d.sg_catoms[t];
-  d.sg_sw[t] = make_float2(d.q2ab_f[a], d.abrw_f[a]);
+  d.sg_sw[t] = make_float4(d.q2ab_f[a], d.abrw_f[a], d.sg_sgam[t], 0.f);
 }
 
 // ------------------------------------------------------------ descriptors
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
     for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
       const int r = k >> 6, c = k & 63;
       if (r < nrows)                  // channel-major: a row's neighbours read adjacent 64-byte slices
-        d.sg_dsdF[((size_t)(c >> 4) * d.n + r0 + r) * 16 + (c & 15)] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
+        d.sg_dsdF[((size_t)(c >> 4) * d.n + r0 + r) * kSgRow + kSgPad + (c & 15)] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
     }
   }
 }
@@ -679,7 +679,7 @@ __global__ void k_sg_mode(Dev d) {
 // the self-volume path sum_i w_i d s_i / d g_a = sum_i w_i sum_k
 // dsdF_i[ch(a)][k] phi_k(d) into acc_ch 6, 7 (the tree derivative pass).
 __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
-  __shared__ float Gs[4][64];               // the row's own ds/dF, per wave
+  __shared__ float Gs[4][4 * kSgRow];       // the row's own ds/dF (padded rows), per wave
   const int rb = xcd_block(blockIdx.x, gridDim.x);
   const int2 blk = d.sg_rows[rb];
   const int sys = d.asys[blk.x];
@@ -695,26 +695,28 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
     const bool aheavy = a < S.a0 + S.nh;
     const int cha = aheavy ? sg_channel(pa.w) : 0;
     const float gama = d.sg_sgam[t];
-    const float *dsdF = d.sg_dsdF + (size_t)cha * d.n * 16;     // channel cha of every slot
+    const float *dsdF = d.sg_dsdF + (size_t)cha * d.n * kSgRow + kSgPad;     // channel cha of every slot
     const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
-    Ga[lane] = d.sg_dsdF[((size_t)(lane >> 4) * d.n + t) * 16 + (lane & 15)];
+    Ga[(lane >> 4) * kSgRow + kSgPad + (lane & 15)] = d.sg_dsdF[((size_t)(lane >> 4) * d.n + t) * kSgRow + kSgPad + (lane & 15)];
+    if (lane < 4 * 2 * kSgPad) Ga[(lane >> 3) * kSgRow + (lane & 7) + ((lane & 7) < kSgPad ? 0 : 16)] = 0.f;
     float sx = 0.f, sy = 0.f, sz = 0.f;                 // d s_a / d x_a
     float gx = 0.f, gy = 0.f, gz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;   // neighbour terms
     float ru = 0.f, rv = 0.f;                           // sum_i {q2ab_i, abrw_i} d s_i / d g_a
     // one stream over the list: heavy neighbours (entries 0 .. nn.x-1; both
     // terms, the second for heavy rows), then, for heavy rows, hydrogens
-    // (entries from the top; the second term only).  The next step's slot
-    // and position are loaded one step ahead.
+    // (entries from the top; the second term only).  The next step's slot,
+    // position and weights are loaded one step ahead.
     const int ntot = nn.x + (aheavy ? nn.y : 0);
     auto entry = [&](int v) { return nb[v < nn.x ? v : kSgNbrCap - 1 - (v - nn.x)]; };
     int jn = lane < ntot ? entry(lane) : t;
-    float4 pn = d.sg_spos[jn];
+    float4 pn = d.sg_spos[jn], wn = d.sg_sw[jn];
     for (int v0 = 0; v0 < ntot; v0 += 64) {
       const int v = v0 + lane;
       const int j = jn;                        // the neighbour's slot
-      const float4 pj = pn;
+      const float4 pj = pn, wj = wn;           // wj: q2ab, abrw, self-volume fraction
       jn = v + 64 < ntot ? entry(v + 64) : t;
       pn = d.sg_spos[jn];
+      wn = d.sg_sw[jn];
       if (v >= ntot) continue;
       const bool jheavy = v < nn.x;
       const float ex = pa.x - pj.x, ey = pa.y - pj.y, ez = pa.z - pj.z;   // x_a - x_j
@@ -724,27 +726,25 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
       float T, Tp, b[4], db[4];
       sg_taper(dist, T, Tp);
       const int ks = sg_basis(dist, b, db);
-      const float *Gc = Ga + sg_channel(pj.w) * 16;
-      const float *Gj = dsdF + (size_t)j * 16;
+      // shells ks .. ks + 3 (ks >= -kSgPad + 1, ks + 3 < 16 + kSgPad): the
+      // zero pads stand for the shells outside 0 .. 15
+      const float *Gc = Ga + sg_channel(pj.w) * kSgRow + kSgPad + ks;
+      float4 gj = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (aheavy) __builtin_memcpy(&gj, dsdF + (size_t)j * kSgRow + ks, sizeof(float4));
+      const float g4[4] = {gj.x, gj.y, gj.z, gj.w};
       float cA = 0.f, cB = 0.f, cU = 0.f;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int kk = ks + q;
-        if (kk < 0 || kk >= kSgNShell) continue;
         const float dphi = db[q] * T + b[q] * Tp;
-        if (jheavy) cA += Gc[kk] * dphi;
-        if (aheavy) {
-          const float g = Gj[kk];
-          cB += g * dphi;
-          cU += g * b[q];
-        }
+        if (jheavy) cA += Gc[q] * dphi;
+        cB += g4[q] * dphi;
+        cU += g4[q] * b[q];
       }
       if (jheavy) {
-        cA *= d.sg_sgam[j] * inv;     // the neighbour's weight in the shell sums
+        cA *= wj.z * inv;             // the neighbour's weight in the shell sums
         sx += cA * ex; sy += cA * ey; sz += cA * ez;
       }
       if (aheavy) {
-        const float2 wj = d.sg_sw[j];
         const float wq = wj.x, wa = wj.y;
         cB *= gama * inv;
         cU *= T;
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
     vx = wave_sum(vx); vy = wave_sum(vy); vz = wave_sum(vz);
     ru = wave_sum(ru); rv = wave_sum(rv);
     if (lane == 0) {
-      const float2 wa2 = d.sg_sw[t];
+      const float4 wa2 = d.sg_sw[t];
       const float qa = wa2.x, ba = wa2.y;
       long long *p = d.acc_ch + 8 * (size_t)a;
       fix_add(p + 0, -(double)(qa * sx + gx));

@@ -683,7 +683,7 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
 // second tier: the roots the first tier queued, with 1536-node levels in
 // dynamic LDS (one block per CU) shared by kBigWaves waves; what still
 // overflows goes to the depth-first kernel
-constexpr int kBigWaves = 4;
+constexpr int kBigWaves = 8;
 template <int CAP, bool PB>
 __global__ __launch_bounds__(64 * kBigWaves) void k_tree_bfs_big(Dev d) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
