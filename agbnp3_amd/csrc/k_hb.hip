@@ -286,6 +286,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
   __shared__ float cgw[kWsWaves][kWsCap][3];
   __shared__ float cgi[kWsWaves][kWsCap][3];
   __shared__ int ckeep[kWsWaves][kWsCap];
+  __shared__ double cspe[kWsWaves][kWsCap];   // spe of the sorted candidates (read in parallel)
   __shared__ double sfree[kWsWaves];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int site = blockIdx.x * kWsWaves + w;
@@ -354,6 +355,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
                         dinv * (aw * W.pos[2] + ak * pk.z)};
     const float xk[3] = {pk.x, pk.y, pk.z};
     cvp[w][e] = s * V;
+    cspe[w][e] = d.spe[k];
     for (int q = 0; q < 3; q++) {
       cgw[w][e][q] = -2.f * aw * V * fp * (W.pos[q] - c[q]);
       cgi[w][e][q] = -2.f * ak * V * fp * (xk[q] - c[q]);
@@ -366,17 +368,16 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
     const bool fixed = (d.options & 1) != 0;
     double freev = vw;
     const double cutoff = (double)kHbSwA0 * vw;
-    int stale = W.stale0;
+    // spe of the previously kept atom (the site's own stale partner first)
+    double stale_sp = W.stale0 >= 0 ? d.spe[W.stale0] : 0.0;
     for (int e = 0; e < cnt; e++) ckeep[w][e] = 0;
     for (int e = 0; e < cnt; e++) {
       if (!(freev > cutoff)) break;
       const float vp = cvp[w][e];
       if (vp > 0.f) {
-        const int k = cs[w][e];
-        const int idx = fixed ? k : stale;
-        const double sp = idx >= 0 ? d.spe[idx] : 0.0;
-        freev -= sp * (double)vp;
-        stale = k;
+        const double spk = cspe[w][e];
+        freev -= (fixed ? spk : stale_sp) * (double)vp;
+        stale_sp = spk;
         ckeep[w][e] = 1;
       }
     }
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(64 * kWsWaves) void k_ws_free(Dev d) {
   for (int e = lane; e < cnt; e += 64) {
     if (!ckeep[w][e]) continue;
     const int k = cs[w][e];
-    const double spk = d.spe[k];
+    const double spk = cspe[w][e];
     for (int q = 0; q < 3; q++) {
       fix_add(&d.acc_hb[3 * (size_t)k + q], -spk * dhw * (double)cgi[w][e][q]);
       jw[q] += -dhw * spk * (double)cgw[w][e][q];
