@@ -232,18 +232,25 @@ __global__ __launch_bounds__(256) void k_sg_features(Dev d) {
     const int nh = over ? 0 : d.sg_nn[i].x;
     const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
     bin[lane] = 0u;
+    // the next step's neighbour position and weight are loaded one step ahead
+    int un = lane < nh ? nb[lane] : t;
+    float4 pn = d.sg_spos[un];
+    float gn = d.sg_sgam[un];
     for (int k0 = 0; k0 < nh; k0 += 64) {
       const int k = k0 + lane;
+      const float4 pj = pn;
+      const float gj = gn;
+      un = k + 64 < nh ? nb[k + 64] : t;
+      pn = d.sg_spos[un];
+      gn = d.sg_sgam[un];
       if (k >= nh) continue;
-      const int u = nb[k];
-      const float4 pj = d.sg_spos[u];
       const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const float d2 = ex * ex + ey * ey + ez * ez;
       if (!(d2 < kSgCut * kSgCut)) continue;
       const float dist = d2 * __builtin_amdgcn_rsqf(d2);
       float T, Tp, b[4], db[4];
       sg_taper(dist, T, Tp);
-      T *= d.sg_sgam[u] * kSgBinScale;
+      T *= gj * kSgBinScale;
       const int k0s = sg_basis(dist, b, db);
       unsigned *f = bin + sg_channel(pj.w) * 16;
 #pragma unroll
