@@ -8,7 +8,9 @@ synthetic solute (seed 20260025), inputs and outputs resident in HBM
 
 Also reported: the roofline position of the dominant kernel (algorithmic
 FP32 flops of its pair work / its average duration from HIP events recorded
-on the instance stream during the timed steps) and the reference CPU path
+on the instance stream over a second run of the same number of steps; the
+headline steps carry no per-kernel events, which cost about 3 % of a step,
+and ms_per_step_with_kernel_events gives that run's rate) and the reference CPU path
 (the reference library built from its own sources, OpenMP) timed on this
 host on a bounded sample of the same workload.
 
@@ -482,8 +484,8 @@ def main():
         step()
     if inst.sync() != api.AGBNP_OK:
         raise RuntimeError("device status error during warm-up")
-    inst.kernel_times()                # clears any earlier records
-    inst.set_options(api.OPT_TIMING)
+    # the timed steps carry no per-kernel events: an event between every two
+    # kernels costs about 3 % of a step
     hip.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -493,14 +495,28 @@ def main():
     t1 = time.perf_counter()
     if rc != api.AGBNP_OK:
         raise RuntimeError("device status error in the timed steps")
-    ktimes = inst.kernel_times()
-    inst.set_options(0)
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = n * args.steps / elapsed
+    # the same number of steps again with a HIP event between kernels on the
+    # instance stream: per-kernel durations for the breakdown and the roofline
+    inst.kernel_times()                # clears any earlier records
+    inst.set_options(api.OPT_TIMING)
+    hip.synchronize()
+    te0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    rc = inst.sync()
+    hip.synchronize()
+    ms_per_step_events = 1e3 * (time.perf_counter() - te0) / args.steps
+    if rc != api.AGBNP_OK:
+        raise RuntimeError("device status error in the event-timed steps")
+    ktimes = inst.kernel_times()
+    inst.set_options(0)
     e_dev = outs["energies"].to_numpy(np.float64, (7,))
     same = all(e_dev[i] == host_out[k] for i, k in enumerate(api.ENERGY_NAMES))
-    log(f"bench: {ms_per_step:.3f} ms/step, {value:.4g} atom-evals/s; device outputs equal host-API call: {same}")
+    log(f"bench: {ms_per_step:.3f} ms/step, {value:.4g} atom-evals/s ({ms_per_step_events:.3f} ms/step with "
+        f"per-kernel events); device outputs equal host-API call: {same}")
     log("bench: kernel ms " + ", ".join(f"{k}={v:.3f}" for k, v in ktimes.items()))
 
     if args.copy_calibration_mb:
@@ -567,6 +583,7 @@ def main():
         roofline=roofline, cpu_baseline=cpu,
         pair_tflops=total_flops / (ms_per_step * 1e-3) / 1e12,
         host_api_atom_evals_per_s=host_rate,
+        ms_per_step_with_kernel_events=ms_per_step_events,
         kernel_ms=ktimes,
         batched=batched,
         surrogate=surrogate,
