@@ -120,8 +120,11 @@ __device__ __forceinline__ int wave_rot(int v) { return __builtin_amdgcn_update_
 
 __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
   __shared__ float colacc[4][4][256];   // [wave][A, gx, gy, gz][column]: fixed-order sum
-  __shared__ float4 cxq[256];           // tile columns: x, y, z, charge
-  __shared__ float2 cbr[256];           // Born radius and its reciprocal
+  // tile columns per 64-column chunk, each chunk twice in a row so that
+  // column (l - s) mod 64 sits at l + 64 - s: x, y, z, charge; Born radius,
+  // its reciprocal
+  __shared__ float4 cxq[4][128];
+  __shared__ float2 cbr[4][128];
   __shared__ double ered[4];
   __shared__ int slot;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -141,8 +144,10 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
         c = make_float4(p.x, p.y, p.z, d.q[j]);
         b = make_float2(bj, __builtin_amdgcn_rcpf(bj));
       }
-      cxq[threadIdx.x] = c;
-      cbr[threadIdx.x] = b;
+      cxq[w][lane] = c;
+      cxq[w][lane + 64] = c;
+      cbr[w][lane] = b;
+      cbr[w][lane + 64] = b;
     }
     float xi = 1e6f + (float)threadIdx.x, yi = 0.f, zi = 0.f, qi = 0.f, bi = 1.f;   // padding: far, no charge
     if (i < tp.y) {
@@ -151,28 +156,31 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
       qi = d.q[i];
       bi = d.br_f[i];
     }
-    const float qib = -0.25f * __builtin_amdgcn_rcpf(bi);   // exp(-d2 / (4 B_i B_j)) = exp(d2 qib / B_j)
+    // exp(-d2 / (4 B_i B_j)) = exp2(d2 qib / B_j)
+    const float qib = -0.25f * 1.44269504088896341f * __builtin_amdgcn_rcpf(bi);
     __syncthreads();
     float rA = 0.f, rx = 0.f, ry = 0.f, rz = 0.f, re = 0.f;
     for (int c = 0; c < nch; c++) {
       float cA = 0.f, cgx = 0.f, cgy = 0.f, cgz = 0.f;
       if (wave_rows) {
-        // the column whose partial sums this lane holds at step s
+        // the column whose partial sums this lane holds at step s: (lane - s) mod 64
+        const float4 *cxc = &cxq[c][lane + 64];
+        const float2 *cbc = &cbr[c][lane + 64];
+#pragma unroll 4
         for (int s = 0; s < 64; s++) {
-          const int col = c * 64 + ((lane - s) & 63);
-          const float4 cp = cxq[col];
-          const float2 cb = cbr[col];
+          const float4 cp = cxc[-s];
+          const float2 cb = cbc[-s];
           const float ex = cp.x - xi, ey = cp.y - yi, ez = cp.z - zi;
           const float d2 = ex * ex + ey * ey + ez * ez;
           // the reference's mask (kernels.c:107): d2 > 0, which also drops the self pair
           const float qq = d2 > 0.f ? qi * cp.w : 0.f;
           const float bb = bi * cb.x;
-          const float et = __expf(d2 * (qib * cb.y));
+          const float et = __builtin_amdgcn_exp2f(d2 * (qib * cb.y));
           const float fgb = __builtin_amdgcn_rsqf(d2 + bb * et);
-          const float fgb3 = fgb * fgb * fgb;
+          const float qf3 = qq * fgb * fgb * fgb;
           re += qq * fgb;
-          const float mw = qq * (1.f - 0.25f * et) * fgb3;
-          const float at = qq * (bb + 0.25f * d2) * et * fgb3;
+          const float mw = qf3 * (1.f - 0.25f * et);
+          const float at = qf3 * et * (bb + 0.25f * d2);
           rx += mw * ex; ry += mw * ey; rz += mw * ez;
           rA += at;
           cA += at;
