@@ -239,9 +239,15 @@ void launch_gbpair(const Dev &d, const Tuning &t, hipStream_t s) {
 // the row side.
 __global__ __launch_bounds__(256) void k_chain(Dev d) {
   __shared__ float2 tab_rc[kTblN], tab_cr[kTblN];   // row <- column, column <- row
-  __shared__ float4 cpos[256];                      // x, y, z, 1/R
-  __shared__ float4 cpar[256];                      // sp (0: H), q2ab, abrw, heavy
-  __shared__ float colacc[8][256];                  // [gb xyz, vw xyz, U, V][column]: waves add in turn
+  // per 64-column chunk: the columns' data, (x, y, z, 1/R) and (sp (0: H),
+  // q2ab, abrw, heavy), and once every wave is past the chunk, in the same
+  // bytes, its column sums [gb xyz, vw xyz, U, V][column] (waves add in turn)
+  __shared__ union {
+    struct {
+      float4 pos[64], par[64];
+    } col;
+    float acc[8][64];
+  } ch[4];
   __shared__ int slot;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
@@ -269,8 +275,8 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
         p = make_float4(pj.x, pj.y, pj.z, 1.0f / pj.w);
         q = make_float4(hv ? d.sp_f[j] : 0.f, d.q2ab_f[j], d.abrw_f[j], hv ? 1.f : 0.f);
       }
-      cpos[threadIdx.x] = p;
-      cpar[threadIdx.x] = q;
+      ch[w].col.pos[lane] = p;
+      ch[w].col.par[lane] = q;
     }
     const int kr = d.cperm[item.x * 256 + threadIdx.x];
     float4 pk = make_float4(-1e6f, -1e6f, -1e6f, 1.f);
@@ -290,8 +296,8 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
       float ca[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       for (int s = 0; s < 64; s++) {
         const int col = c * 64 + ((lane - s) & 63);        // the column whose sums this lane holds
-        const float4 pj = cpos[col];
-        const float4 pp = cpar[col];
+        const float4 pj = ch[c].col.pos[col & 63];
+        const float4 pp = ch[c].col.par[col & 63];
         const bool jheavy = pp.w != 0.f;
         const bool pair = (kr >= 0) && (kheavy || jheavy) && !(diag && col == w * 64 + lane);
         if (pair) {
@@ -329,11 +335,13 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
 #pragma unroll
         for (int q = 0; q < 8; q++) ca[q] = wave_rot(ca[q]);
       }
-      // this chunk's column sums over the four waves in a fixed order (0 + 1 + 2 + 3)
+      // this chunk's column sums over the four waves in a fixed order (0 + 1 + 2 + 3),
+      // over its column data once every wave is done with it
+      __syncthreads();
       for (int ww = 0; ww < 4; ww++) {
         if (w == ww) {
 #pragma unroll
-          for (int q = 0; q < 8; q++) colacc[q][c * 64 + lane] = ww == 0 ? ca[q] : colacc[q][c * 64 + lane] + ca[q];
+          for (int q = 0; q < 8; q++) ch[c].acc[q][lane] = ww == 0 ? ca[q] : ch[c].acc[q][lane] + ca[q];
         }
         __syncthreads();
       }
@@ -350,9 +358,8 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
     const int j = d.cperm[item.y * 256 + threadIdx.x];
     if (!diag && j >= 0) {
       long long *pc = d.acc_ch + 8 * (size_t)j;
-      const int t = threadIdx.x;
 #pragma unroll
-      for (int q = 0; q < 8; q++) fix_add(pc + q, (double)colacc[q][t]);
+      for (int q = 0; q < 8; q++) fix_add(pc + q, (double)ch[w].acc[q][lane]);   // column w * 64 + lane
     }
   }
 }
