@@ -206,6 +206,8 @@ struct Dev {
   int *tree_ovf;                       // [4][nh] those roots' heavy rows
   double *volumep_raw, *sa_raw, *sa_f, *sp, *spe, *psvol, *gammap;   // [n]
   float *sp_f, *spe_f;                 // FP32 copies for pair kernels
+  float4 *dcol;                        // heavy atoms as descreening columns: x, y, z, 1/R
+  float2 *dcw;                         // and (sp / R, radius type as int bits)
   double *beta, *br, *brw, *swf;       // [n]
   float *br_f;
   double *gb_epart;                    // per GB energy slot: sum of q_i q_j / f

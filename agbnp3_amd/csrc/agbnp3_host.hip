@@ -655,7 +655,7 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
       dalloc(L, &d.acc_gbA, nn) || dalloc(L, &d.acc_gbg, 3 * nn) || dalloc(L, &d.acc_ch, 8 * nn) ||
       dalloc(L, &d.volumep_raw, nn) || dalloc(L, &d.sa_raw, nn) || dalloc(L, &d.sa_f, nn) ||
       dalloc(L, &d.sp, nn) || dalloc(L, &d.spe, nn) || dalloc(L, &d.psvol, nn) || dalloc(L, &d.gammap, nn) ||
-      dalloc(L, &d.sp_f, nn) || dalloc(L, &d.spe_f, nn) || dalloc(L, &d.beta, nn) || dalloc(L, &d.br, nn) ||
+      dalloc(L, &d.sp_f, nn) || dalloc(L, &d.spe_f, nn) || dalloc(L, &d.dcol, nn) || dalloc(L, &d.dcw, nn) || dalloc(L, &d.beta, nn) || dalloc(L, &d.br, nn) ||
       dalloc(L, &d.brw, nn) || dalloc(L, &d.swf, nn) || dalloc(L, &d.br_f, nn) ||
       dalloc(L, &d.gb_epart, (size_t)std::max(tile0, 1)) || dalloc(L, &d.A, nn) || dalloc(L, &d.q2ab, nn) ||
       dalloc(L, &d.abrw, nn) || dalloc(L, &d.q2ab_f, nn) || dalloc(L, &d.abrw_f, nn) ||

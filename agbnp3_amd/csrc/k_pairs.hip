@@ -37,22 +37,27 @@ __device__ __forceinline__ void stage_tables(const Dev &d, float2 *dst, int t, b
 // Q4 = f(d/R_j)/R_j from table [rtype_i][rtype_j]
 // (reference agbnp3_inverse_born_radii_nolist_soa, kernels.c:500-678)
 template <int ROWS>
-__global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
+__global__ __launch_bounds__(ROWS) void k_descreen(Dev d, const float4 *__restrict__ ccol,
+                                                  const float2 *__restrict__ cwt) {
   extern __shared__ float2 dyn_tab[];     // [ntypes][512]: tables [t_row][*]
   float2 (*tab)[kTblN] = reinterpret_cast<float2 (*)[kTblN]>(dyn_tab);
-  __shared__ float4 cpos[kTile];     // x, y, z, 1/R
-  __shared__ float2 cw[kTile];       // sp/R, rtype (as float bits)
   __shared__ int slot;
   if (d.sg_on && !*d.sg_any_plain) return;   // every system on the surrogate this call (grid-uniform)
   const float dx_ = d.tdx, dxinv = d.tdxinv, dp1 = dx_ / 6.0f, dp2 = dx_ * dp1;
+  // a column is the same for every lane: its record (ccol, cwt: d.dcol and
+  // d.dcw, written by k_scale) comes through scalar loads, its table base is
+  // scalar arithmetic
   int staged = -1;
-  for (int it = next_item(&d.queue[Q_DESC], &slot); it < d.ndesc_items; it = next_item(&d.queue[Q_DESC], &slot)) {
+  for (int it0 = next_item(&d.queue[Q_DESC], &slot); it0 < d.ndesc_items;
+       it0 = next_item(&d.queue[Q_DESC], &slot)) {
+    const int it = __builtin_amdgcn_readfirstlane(it0);
     const int4 item = d.desc_items[it];
     if (d.sg_on && d.sg_state[4 * d.rblk[item.x].y + 2] == 1) continue;   // surrogate call for this system
     const int t = d.rblk[item.x].x;
     if (t != staged) {                 // uniform: every thread reads the same item
       stage_tables<ROWS>(d, dyn_tab, t, true);
       staged = t;
+      __syncthreads();
     }
     const int i = d.row_perm[item.x * ROWS + threadIdx.x];
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -60,31 +65,30 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d) {
     const int c0 = item.y, c1 = item.z;
     double acc = 0.0;
     for (int base = c0; base < c1; base += kTile) {
-      __syncthreads();
-      for (int k = threadIdx.x; k < kTile; k += ROWS) {
-        const int j = base + k;
-        if (j < c1) {
-          const float4 pj = d.pos[j];
-          const float iR = 1.0f / pj.w;
-          cpos[k] = make_float4(pj.x, pj.y, pj.z, iR);
-          cw[k] = make_float2(d.sp_f[j] * iR, __int_as_float(d.rtype[j]));
-        }
-      }
-      __syncthreads();
       const int nc = min(kTile, c1 - base);
       float tacc = 0.f;
-      if (i >= 0) {
-#pragma unroll 4
-        for (int c = 0; c < nc; c++) {                // independent columns overlap; sums stay in order
-          const float4 pj = cpos[c];
-          const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-          const float dist = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);   // hardware square root (1 ulp)
-          const float2 w = cw[c];
-          float q, dq;
-          q4_lookup<true, false>(tab[__float_as_int(w.y)], dist * pj.w, dx_, dxinv, dp1, dp2, 1.f, q, dq);
-          tacc += (base + c != i) ? q * w.x : 0.f;
+      // every lane runs the loop (a padding row, i < 0, at the origin: its sum
+      // is dropped), so the column index stays wave-uniform
+      auto column = [&](int j, const float4 &pj, const float2 &w) {
+        const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        const float dist = __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);   // hardware square root (1 ulp)
+        float q, dq;
+        q4_lookup<true, false>(tab[__float_as_int(w.y)], dist * pj.w, dx_, dxinv, dp1, dp2, 1.f, q, dq);
+        tacc += (j != i) ? q * w.x : 0.f;
+      };
+      int c = 0;
+      for (; c + 4 <= nc; c += 4) {                   // four columns' records in two scalar loads
+        float4 pj[4];
+        float2 w[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          pj[u] = ccol[base + c + u];
+          w[u] = cwt[base + c + u];
         }
+#pragma unroll
+        for (int u = 0; u < 4; u++) column(base + c + u, pj[u], w[u]);   // sums stay in column order
       }
+      for (; c < nc; c++) column(base + c, ccol[base + c], cwt[base + c]);
       acc += (double)tacc;
     }
     if (i >= 0) fix_add(&d.acc_beta[i], acc);
@@ -96,9 +100,11 @@ void launch_descreen(const Dev &d, const Tuning &t, hipStream_t s) {
   const size_t lds = (size_t)d.ntypes * kTblN * sizeof(float2);
   int grid = t.desc_grid > 0 ? std::min(t.desc_grid, d.ndesc_items) : d.ndesc_items;
   if (d.sg_on) grid = std::min(grid, kSgPersistentGrid);   // calls where every block leaves at once
-  if (d.rows == 64) hipLaunchKernelGGL(k_descreen<64>, dim3(grid), dim3(64), lds, s, d);
-  else if (d.rows == 128) hipLaunchKernelGGL(k_descreen<128>, dim3(grid), dim3(128), lds, s, d);
-  else hipLaunchKernelGGL(k_descreen<256>, dim3(grid), dim3(256), lds, s, d);
+  const float4 *ccol = d.dcol;
+  const float2 *cwt = d.dcw;
+  if (d.rows == 64) hipLaunchKernelGGL(k_descreen<64>, dim3(grid), dim3(64), lds, s, d, ccol, cwt);
+  else if (d.rows == 128) hipLaunchKernelGGL(k_descreen<128>, dim3(grid), dim3(128), lds, s, d, ccol, cwt);
+  else hipLaunchKernelGGL(k_descreen<256>, dim3(grid), dim3(256), lds, s, d, ccol, cwt);
 }
 
 // ------------------------------------------------------------- GB pairs
