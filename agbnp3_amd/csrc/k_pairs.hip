@@ -309,7 +309,7 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
         if (pair) {
           const float ex = pj.x - pk.x, ey = pj.y - pk.y, ez = pj.z - pk.z;
           const float d2 = ex * ex + ey * ey + ez * ez;
-          const float id = rsqrtf(d2), dist = d2 * id;
+          const float id = __builtin_amdgcn_rsqf(d2), dist = d2 * id;   // hardware reciprocal square root (1 ulp)
           float hgb = 0.f, hvw = 0.f;
           if (jheavy) {                                      // k descreened by j
             float qv, dq;
