@@ -81,7 +81,8 @@ def profiled_traffic(kernel):
     except (OSError, KeyError, ValueError):
         return None
     for name, k in kernels.items():
-        if name == "k_" + kernel or name.startswith("k_" + kernel + "<"):
+        base = name.split("(")[0]            # "k_chain(Dev)" -> "k_chain"
+        if base == "k_" + kernel or base.startswith("k_" + kernel + "<"):
             return k["hbm_bytes"]
     return None
 
