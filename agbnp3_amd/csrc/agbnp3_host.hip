@@ -178,6 +178,11 @@ struct Pack {
   std::vector<int> rtype;              // per atom
   std::vector<char> heavy;             // per atom
   hipStream_t stream = nullptr;
+  // second stream for the water-site kernels, which need only the self
+  // volumes: they run beside the descreening and pair kernels (fork and join
+  // events, also inside a captured graph)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void *> allocs, item_allocs;
   Dev d{};
   double *dx = nullptr, *dy = nullptr, *dz = nullptr;      // coordinate staging (device)
@@ -331,6 +336,11 @@ void free_pack(Pack &P) {
   P.ev_sets.clear();
   if (P.stream) (void)hipStreamDestroy(P.stream);
   P.stream = nullptr;
+  if (P.side) (void)hipStreamDestroy(P.side);
+  P.side = nullptr;
+  if (P.ev_fork) (void)hipEventDestroy(P.ev_fork);
+  if (P.ev_join) (void)hipEventDestroy(P.ev_join);
+  P.ev_fork = P.ev_join = nullptr;
 }
 
 // static water-site topology: geometry per HB-active atom, following the
@@ -670,6 +680,9 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
   HIPCHK(hipMemset(d.nbl_state, 0, sizeof(int) * 4));
   HIPCHK(hipHostMalloc((void **)&P.hstage, (18 * nn + 7 * (size_t)P.nsys) * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
   HIPCHK(prepare_pairs(P.ntypes));
   HIPCHK(prepare_tree());
   return build_items(P);
@@ -715,6 +728,15 @@ int run_pipeline(Pack &P, const double *x, const double *y, const double *z, dou
   mark("tree_volumes");
   launch_scale(d, s);
   mark("scale");
+  // with per-kernel timing every kernel stays on the one stream
+  const bool fork = !timed && P.side && d.nws > 0;
+  if (fork) {
+    HIPCHK(hipEventRecord(P.ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
+    launch_ws_place(d, P.side);
+    launch_ws_free(d, P.side);
+    HIPCHK(hipEventRecord(P.ev_join, P.side));
+  }
   launch_sg_front(d, s);
   mark("sg_front");
   launch_descreen(d, t, s);
@@ -730,10 +752,14 @@ int run_pipeline(Pack &P, const double *x, const double *y, const double *z, dou
   mark("chain");
   launch_sg_grad(d, s);
   mark("sg_grad");
-  launch_ws_place(d, s);
-  mark("ws_place");
-  launch_ws_free(d, s);
-  mark("ws_free");
+  if (fork) {
+    HIPCHK(hipStreamWaitEvent(s, P.ev_join, 0));
+  } else {
+    launch_ws_place(d, s);
+    mark("ws_place");
+    launch_ws_free(d, s);
+    mark("ws_free");
+  }
   launch_uv(d, s);
   mark("uv");
   launch_tree(d, 1, s);
