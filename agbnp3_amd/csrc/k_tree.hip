@@ -29,6 +29,8 @@
 // surface-area changes through gamma', dervs, derus).
 #include <float.h>
 
+#include <type_traits>
+
 #include "agb_common.h"
 #include "agb_device.h"
 
@@ -404,8 +406,14 @@ struct BfsLds {
   // two node buffers: Gaussian, member slots (16 bits each, root excluded),
   // end of the node's sibling group, parent index in the previous order
   float ga[2][LC], gp[2][LC], gx[2][LC], gy[2][LC], gz[2][LC];
-  unsigned long long path[2][LC];
-  int gend[2][LC], gpar[2][LC];
+  // member slots packed 8 bits each while slots fit (CAP < 256), else 16;
+  // node indices in 8 bits while levels hold fewer than 256 nodes
+  using PathT = typename std::conditional<(CAP < 256), unsigned, unsigned long long>::type;
+  using NodeT = typename std::conditional<(LC < 256), unsigned char, unsigned short>::type;
+  static constexpr int kSlotBits = CAP < 256 ? 8 : 16;
+  static constexpr PathT kSlotMask = (PathT)((1u << kSlotBits) - 1u);
+  PathT path[2][LC];
+  NodeT gend[2][LC], gpar[2][LC];
   int pref[LC + 1], cnt[LC];
   int wcnt[2][W];                       // kept candidates per wave of one pass
 };
@@ -491,7 +499,7 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC, W> &S, int cb, int 
       }
       uu = lo;
       const int vv = uu + 1 + (q - S.pref[uu]);
-      sb = (int)((S.path[cb][vv] >> (16 * (N - 3))) & 0xFFFFull);    // last atom of sibling v
+      sb = (int)((S.path[cb][vv] >> (S.kSlotBits * (N - 3))) & S.kSlotMask);    // last atom of sibling v
       const Gauss par = {S.ga[cb][uu], S.gp[cb][uu], S.gx[cb][uu], S.gy[cb][uu], S.gz[cb][uu]};
       const float4 pb = S.p[sb];
       o = overlap(par, pb.w, pb.x, pb.y, pb.z);
@@ -501,19 +509,19 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC, W> &S, int cb, int 
     int total;
     const int pos = nn + pass_compact<W>(S.wcnt, parity, __ballot(keep), total);
     if (keep) {
-      const unsigned long long pu = S.path[cb][uu];
+      const auto pu = S.path[cb][uu];
       int mem[N];
       mem[0] = 0;
 #pragma unroll
-      for (int k = 0; k < N - 2; k++) mem[1 + k] = (int)((pu >> (16 * k)) & 0xFFFFull);
+      for (int k = 0; k < N - 2; k++) mem[1 + k] = (int)((pu >> (S.kSlotBits * k)) & S.kSlotMask);
       mem[N - 1] = sb;
       add_contrib<N, CAP, PB>(S, o, mem, root, df);
       if (STORE) {
         if (pos < LC) {
           S.ga[nb][pos] = o.a; S.gp[nb][pos] = o.p;
           S.gx[nb][pos] = o.cx; S.gy[nb][pos] = o.cy; S.gz[nb][pos] = o.cz;
-          S.path[nb][pos] = pu | ((unsigned long long)sb << (16 * (N - 2)));
-          S.gpar[nb][pos] = uu;
+          S.path[nb][pos] = pu | ((decltype(pu))sb << (S.kSlotBits * (N - 2)));
+          S.gpar[nb][pos] = (std::remove_reference_t<decltype(S.gpar[0][0])>)uu;
         }
         atomicAdd(&S.cnt[uu], 1);
       }
@@ -529,7 +537,7 @@ __device__ __forceinline__ int bfs_order(BfsLds<CAP, PB, LC, W> &S, int cb, int 
   __syncthreads();
   for (int k = tid; k < nn; k += NT) {
     const int par = S.gpar[nb][k];
-    S.gend[nb][k] = S.pref[par] + S.cnt[par];
+    S.gend[nb][k] = (std::remove_reference_t<decltype(S.gend[0][0])>)(S.pref[par] + S.cnt[par]);
   }
   __syncthreads();
   return nn;
@@ -593,14 +601,14 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, in
         if (pos < LC) {
           S.ga[0][pos] = o.a; S.gp[0][pos] = o.p;
           S.gx[0][pos] = o.cx; S.gy[0][pos] = o.cy; S.gz[0][pos] = o.cz;
-          S.path[0][pos] = (unsigned long long)sb;
+          S.path[0][pos] = (std::remove_reference_t<decltype(S.path[0][0])>)sb;
         }
       }
       n += total;
     }
     nkept += (unsigned)n;
     if (n > LC) ovf = true;
-    for (int k = tid; k < n && k < LC; k += NT) S.gend[0][k] = n;
+    for (int k = tid; k < n && k < LC; k += NT) S.gend[0][k] = (std::remove_reference_t<decltype(S.gend[0][0])>)n;
     __syncthreads();
   }
   if (!ovf && n > 1) {
