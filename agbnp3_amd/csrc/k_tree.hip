@@ -682,9 +682,14 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   // roots in runs of `run` per queue grab when there are many per block
   // (one returning atomic on the queue head per grab)
   const int run = max(1, min(8, d.nh / (int)(gridDim.x * 8)));
+  // queue position k takes root (k * P) mod nh, P a prime that does not
+  // divide nh: roots in flight together lie far apart in the atom order, so
+  // their fixed-point scatters rarely meet on one atom
+  const long long P = d.nh % 7919 ? 7919 : (d.nh % 7927 ? 7927 : 7933);
   unsigned long long tk = 0, te = 0;
-  for (int h0 = next_item_n(head, &slot, run); h0 < d.nh; h0 = next_item_n(head, &slot, run))
-    for (int h = h0; h < min(h0 + run, d.nh); h++) tree_bfs_root<CAP, PB, kLevelCap, 1>(d, S, h, 0, tk, te);
+  for (int k0 = next_item_n(head, &slot, run); k0 < d.nh; k0 = next_item_n(head, &slot, run))
+    for (int k = k0; k < min(k0 + run, d.nh); k++)
+      tree_bfs_root<CAP, PB, kLevelCap, 1>(d, S, (int)(((long long)k * P) % d.nh), 0, tk, te);
   add_tree_counts(d, PB, tk, te);
 }
 
