@@ -242,12 +242,16 @@ __global__ __launch_bounds__(256) void k_energy(Dev d, double *energies) {
   };
   int b0, b1;
   range(S.nblk, b0, b1);
+  // (the loops unrolled so that several loads are in flight; each sum keeps its order)
   for (int k = 0; k < EP_PAIR; k++)
+#pragma unroll 8
     for (int b = b0; b < b1; b++) acc[k] += d.epart[(size_t)k * d.nablk + S.blk0 + b];
   range(S.ntile, b0, b1);
   // tile partials hold each unordered pair once; E_pair = 2 df sum_{i<j} q_i q_j / f
+#pragma unroll 8
   for (int b = b0; b < b1; b++) acc[EP_PAIR] += 2.0 * S.df64 * d.gb_epart[S.tile0 + b];
   range(S.nws, b0, b1);
+#pragma unroll 8
   for (int b = b0; b < b1; b++) acc[EP_HB] += d.ws_part[S.ws0 + b];
   for (int k = 0; k < EP_COUNT; k++) {
     const double s = wave_sum(acc[k]);

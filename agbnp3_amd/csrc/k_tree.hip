@@ -679,9 +679,9 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   __shared__ BfsLds<CAP, PB, kLevelCap, 1> S;
   __shared__ int slot;
   unsigned *head = &d.queue[PB ? Q_TREE_B : Q_TREE_A];
-  // roots in runs of `run` per queue grab when there are many per block
-  // (one returning atomic on the queue head per grab)
-  const int run = max(1, min(8, d.nh / (int)(gridDim.x * 8)));
+  // roots in runs of `run` per queue grab (one returning atomic on the queue
+  // head per grab; at least two: the grabs serialise on that one address)
+  const int run = max(2, min(8, d.nh / (int)(gridDim.x * 8)));
   // queue position k takes root (k * P) mod nh, P a prime that does not
   // divide nh: roots in flight together lie far apart in the atom order, so
   // their fixed-point scatters rarely meet on one atom
