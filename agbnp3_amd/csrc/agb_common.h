@@ -22,6 +22,12 @@ constexpr double kTokcal = 332.0;
 constexpr double kRwNp = 1.4;           // water radius in the vdW term
 constexpr float kHbLength = 2.5f, kHbRadius = 1.4f;
 constexpr float kHbSwA = 0.65f, kHbSwB = 1.0f, kHbSwA0 = 0.6f;
+// per-call device counters: [0..7] scalars (see agbnp3_get_counts); the
+// overlap-tree kept / evaluated totals spread over kCountSlots 64-byte lines
+// (kCountBase + 8 s, + 1), so the blocks' adds at exit do not queue on one
+// address
+constexpr int kCountSlots = 32, kCountBase = 8;
+constexpr int kCounters = kCountBase + 8 * kCountSlots;
 constexpr int kTblN = 512;              // spline nodes per table (y[511] = 0)
 constexpr float kTblAmax = 20.0f;
 

@@ -658,7 +658,7 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
       dalloc(L, &d.cell_tmp, nn) || dalloc(L, &d.nne, nhh) || dalloc(L, &d.nbr, nhh * 256) ||
       dalloc(L, &d.cand_nne, nhh) || dalloc(L, &d.cand_nbr, nhh * kCandCap) || dalloc(L, &d.ref_pos, nn) ||
       dalloc(L, &d.nbl_state, 4) ||
-      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, Q_COUNT) || dalloc(L, &d.counters, 8) ||
+      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, Q_COUNT) || dalloc(L, &d.counters, kCounters) ||
       dalloc(L, &d.tree_ovf, 4 * nhh) ||
       dalloc(L, &d.acc_vol, nn) || dalloc(L, &d.acc_sa, nn) || dalloc(L, &d.acc_grad, nn * TG_N) ||
       dalloc(L, &d.acc_hb, 3 * nn) || dalloc(L, &d.acc_derh, nn) || dalloc(L, &d.acc_beta, nn) ||
@@ -1407,11 +1407,14 @@ int agbnp3_get_internal(int tag, const char *name, double *out) {
 int agbnp3_get_counts(int tag, long long *counts) {
   Instance *I = get_instance(tag, "agbnp3_get_counts()");
   if (!I) return AGBNP_ERR;
-  unsigned long long c[8];
+  unsigned long long c[kCounters];
   HIPCHK(hipStreamSynchronize(I->p.stream));
   HIPCHK(hipMemcpy(c, I->p.d.counters, sizeof(c), hipMemcpyDeviceToHost));
-  counts[0] = (long long)c[0];
-  counts[1] = (long long)c[1];
+  counts[0] = counts[1] = 0;
+  for (int k = 0; k < kCountSlots; k++) {
+    counts[0] += (long long)c[kCountBase + 8 * k];
+    counts[1] += (long long)c[kCountBase + 8 * k + 1];
+  }
   counts[2] = I->m.nws;
   counts[3] = (long long)c[2];
   counts[4] = (long long)c[3];

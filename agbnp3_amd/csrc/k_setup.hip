@@ -41,6 +41,10 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
 #pragma unroll
     for (int q = 0; q < 8; q++) d.acc_ch[8 * (size_t)i + q] = 0;
   }
+  if (i < kCountSlots) {
+    d.counters[kCountBase + 8 * i] = 0;
+    d.counters[kCountBase + 8 * i + 1] = 0;
+  }
   if (i == 0) {
     *d.status = 0;
     for (int k = 0; k < 8; k++) d.counters[k] = 0;

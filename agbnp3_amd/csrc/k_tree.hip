@@ -380,8 +380,9 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
   for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
     tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(size_t)(2 + (PB ? 1 : 0)) * d.nh + k], tk, te);
   if (!PB && threadIdx.x == 0 && (tk | te)) {
-    atomicAdd(&d.counters[0], tk);
-    atomicAdd(&d.counters[1], te);
+    unsigned long long *c = d.counters + kCountBase + 8 * (blockIdx.x % kCountSlots);
+    atomicAdd(c, tk);
+    atomicAdd(c + 1, te);
   }
 }
 
@@ -667,8 +668,9 @@ __device__ void tree_bfs_root(const Dev &d, BfsLds<CAP, PB, LC, W> &S, int h, in
 __device__ __forceinline__ void add_tree_counts(const Dev &d, bool pass_b, unsigned long long tk,
                                                 unsigned long long te) {
   if (pass_b || (threadIdx.x & 63) != 0) return;
-  if (threadIdx.x == 0 && tk) atomicAdd(&d.counters[0], tk);
-  if (te) atomicAdd(&d.counters[1], te);
+  unsigned long long *c = d.counters + kCountBase + 8 * (blockIdx.x % kCountSlots);
+  if (threadIdx.x == 0 && tk) atomicAdd(c, tk);
+  if (te) atomicAdd(c + 1, te);
 }
 
 
