@@ -42,17 +42,22 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // ------------------------------------------------------------ work queues
 // Persistent blocks take work items from a queue head in memory; every block
 // leaves once the head has passed the item count, so the grid always drains.
-// `slot` is block-shared scratch for the broadcast.
+// A block's first item (or run of n items) is its own index: the whole grid
+// starting at once would otherwise queue on the head's one address.  Later
+// items come from the head, offset past those.  `slot` is block-shared
+// scratch for the broadcast.
+__device__ __forceinline__ int first_item() { return (int)blockIdx.x; }
 __device__ __forceinline__ int next_item(unsigned *head, int *slot) {
   __syncthreads();                       // the previous item is finished by all waves
-  if (threadIdx.x == 0) *slot = (int)atomicAdd(head, 1u);
+  if (threadIdx.x == 0) *slot = (int)(gridDim.x + atomicAdd(head, 1u));
   __syncthreads();
   return *slot;
 }
 // the same, taking `n` consecutive items per grab
+__device__ __forceinline__ int first_item_n(int n) { return (int)blockIdx.x * n; }
 __device__ __forceinline__ int next_item_n(unsigned *head, int *slot, int n) {
   __syncthreads();
-  if (threadIdx.x == 0) *slot = (int)atomicAdd(head, (unsigned)n);
+  if (threadIdx.x == 0) *slot = (int)(gridDim.x * (unsigned)n + atomicAdd(head, (unsigned)n));
   __syncthreads();
   return *slot;
 }

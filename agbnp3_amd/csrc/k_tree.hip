@@ -377,7 +377,7 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
   const int count = (int)d.counters[6 + (PB ? 1 : 0)];
   unsigned *head = &d.queue[PB ? Q_TREE_FB : Q_TREE_FA];
   unsigned long long tk = 0, te = 0;
-  for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
+  for (int k = first_item(); k < count; k = next_item(head, &slot))
     tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(size_t)(2 + (PB ? 1 : 0)) * d.nh + k], tk, te);
   if (!PB && threadIdx.x == 0 && (tk | te)) {
     unsigned long long *c = d.counters + kCountBase + 8 * (blockIdx.x % kCountSlots);
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   // their fixed-point scatters rarely meet on one atom
   const long long P = d.nh % 7919 ? 7919 : (d.nh % 7927 ? 7927 : 7933);
   unsigned long long tk = 0, te = 0;
-  for (int k0 = next_item_n(head, &slot, run); k0 < d.nh; k0 = next_item_n(head, &slot, run))
+  for (int k0 = first_item_n(run); k0 < d.nh; k0 = next_item_n(head, &slot, run))
     for (int k = k0; k < min(k0 + run, d.nh); k++)
       tree_bfs_root<CAP, PB, kLevelCap, 1>(d, S, (int)(((long long)k * P) % d.nh), 0, tk, te);
   add_tree_counts(d, PB, tk, te);
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(64 * kBigWaves) void k_tree_bfs_big(Dev d) {
   const int count = (int)d.counters[4 + (PB ? 1 : 0)];
   unsigned *head = &d.queue[PB ? Q_TREE_BB : Q_TREE_BA];
   unsigned long long tk = 0, te = 0;
-  for (int k = next_item(head, &slot); k < count; k = next_item(head, &slot))
+  for (int k = first_item(); k < count; k = next_item(head, &slot))
     tree_bfs_root<CAP, PB, kLevelCapBig, kBigWaves>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1, tk, te);
   add_tree_counts(d, PB, tk, te);
 }
