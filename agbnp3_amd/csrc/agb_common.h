@@ -143,6 +143,10 @@ struct SgTrain {
 
 // work-queue heads (cleared by the ingest kernel of every call)
 enum : int { Q_DESC = 0, Q_GB, Q_CHAIN, Q_TREE_A, Q_TREE_B, Q_TREE_BA, Q_TREE_BB, Q_TREE_FA, Q_TREE_FB, Q_COUNT };
+// each queue has kQueueSpread heads, kQueueStride uints (64 bytes) apart: a
+// block takes its later items from head blockIdx mod kQueueSpread
+constexpr int kQueueSpread = 8, kQueueStride = 16;
+constexpr int kQueueWords = Q_COUNT * kQueueSpread * kQueueStride;
 
 // Device view of a pack of one or more systems, passed by value to every kernel.
 struct Dev {
@@ -179,7 +183,7 @@ struct Dev {
   int ncblk;
   int ndesc_items, nchain_items, ngb_items;
   const int *gb_eslot;   // energy slot of each GB item
-  unsigned *queue;       // [Q_COUNT]
+  unsigned *queue;       // [Q_COUNT][kQueueSpread][kQueueStride] (heads at stride 0)
   const int2 *ablk;      // atom blocks: [start, end) inside one system
   int nablk;
   const WsDef *wsdef;

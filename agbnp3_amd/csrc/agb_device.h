@@ -46,18 +46,30 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // starting at once would otherwise queue on the head's one address.  Later
 // items come from the head, offset past those.  `slot` is block-shared
 // scratch for the broadcast.
+// The later items are spread over S = min(kQueueSpread, gridDim) heads
+// (every head has a block): head x hands out items gridDim + S c + x
+// (c = 0, 1, ...) to the blocks with blockIdx mod S = x.
+__device__ __forceinline__ unsigned *queue_head(unsigned *queue, int which) {
+  return queue + which * kQueueSpread * kQueueStride;
+}
 __device__ __forceinline__ int first_item() { return (int)blockIdx.x; }
 __device__ __forceinline__ int next_item(unsigned *head, int *slot) {
   __syncthreads();                       // the previous item is finished by all waves
-  if (threadIdx.x == 0) *slot = (int)(gridDim.x + atomicAdd(head, 1u));
+  if (threadIdx.x == 0) {
+    const unsigned S = min((unsigned)kQueueSpread, gridDim.x), x = blockIdx.x % S;
+    *slot = (int)(gridDim.x + S * atomicAdd(head + x * kQueueStride, 1u) + x);
+  }
   __syncthreads();
   return *slot;
 }
-// the same, taking `n` consecutive items per grab
+// the same in runs of `n` consecutive items (run u = items u n .. u n + n - 1)
 __device__ __forceinline__ int first_item_n(int n) { return (int)blockIdx.x * n; }
 __device__ __forceinline__ int next_item_n(unsigned *head, int *slot, int n) {
   __syncthreads();
-  if (threadIdx.x == 0) *slot = (int)(gridDim.x * (unsigned)n + atomicAdd(head, (unsigned)n));
+  if (threadIdx.x == 0) {
+    const unsigned S = min((unsigned)kQueueSpread, gridDim.x), x = blockIdx.x % S;
+    *slot = (int)((gridDim.x + S * atomicAdd(head + x * kQueueStride, 1u) + x) * (unsigned)n);
+  }
   __syncthreads();
   return *slot;
 }

@@ -658,7 +658,7 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
       dalloc(L, &d.cell_tmp, nn) || dalloc(L, &d.nne, nhh) || dalloc(L, &d.nbr, nhh * 256) ||
       dalloc(L, &d.cand_nne, nhh) || dalloc(L, &d.cand_nbr, nhh * kCandCap) || dalloc(L, &d.ref_pos, nn) ||
       dalloc(L, &d.nbl_state, 4) ||
-      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, Q_COUNT) || dalloc(L, &d.counters, kCounters) ||
+      dalloc(L, &d.status, 1) || dalloc(L, &d.queue, kQueueWords) || dalloc(L, &d.counters, kCounters) ||
       dalloc(L, &d.tree_ovf, 4 * nhh) ||
       dalloc(L, &d.acc_vol, nn) || dalloc(L, &d.acc_sa, nn) || dalloc(L, &d.acc_grad, nn * TG_N) ||
       dalloc(L, &d.acc_hb, 3 * nn) || dalloc(L, &d.acc_derh, nn) || dalloc(L, &d.acc_beta, nn) ||

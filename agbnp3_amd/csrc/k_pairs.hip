@@ -49,7 +49,7 @@ __global__ __launch_bounds__(ROWS) void k_descreen(Dev d, const float4 *__restri
   // scalar arithmetic
   int staged = -1;
   for (int it0 = first_item(); it0 < d.ndesc_items;
-       it0 = next_item(&d.queue[Q_DESC], &slot)) {
+       it0 = next_item(queue_head(d.queue, Q_DESC), &slot)) {
     const int it = __builtin_amdgcn_readfirstlane(it0);
     const int4 item = d.desc_items[it];
     if (d.sg_on && d.sg_state[4 * d.rblk[item.x].y + 2] == 1) continue;   // surrogate call for this system
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_gbpair(Dev d) {
   __shared__ double ered[4];
   __shared__ int slot;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int it = first_item(); it < d.ngb_items; it = next_item(&d.queue[Q_GB], &slot)) {
+  for (int it = first_item(); it < d.ngb_items; it = next_item(queue_head(d.queue, Q_GB), &slot)) {
     const int4 tp = d.gb_items[it];      // rows [x, y), columns [z, w)
     const bool diag = tp.x == tp.z;
     const int i = tp.x + threadIdx.x;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_chain(Dev d) {
   const float sgn2 = (d.options & 2) ? -1.f : 1.f;
   if (d.sg_on && !*d.sg_any_plain) return;   // every system on the surrogate this call (grid-uniform)
   for (int it = first_item(); it < d.nchain_items;
-       it = next_item(&d.queue[Q_CHAIN], &slot)) {
+       it = next_item(queue_head(d.queue, Q_CHAIN), &slot)) {
     const int4 item = d.chain_items[it];
     const int4 bp = d.cblk[item.x], bq = d.cblk[item.y];
     if (d.sg_on && d.sg_state[4 * bp.y + 2] == 1) continue;   // surrogate call: k_sg_grad instead

@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
 #pragma unroll
     for (int q = 0; q < 8; q++) d.acc_ch[8 * (size_t)i + q] = 0;
   }
+  if (i < Q_COUNT * kQueueSpread) d.queue[i * kQueueStride] = 0;
   if (i < kCountSlots) {
     d.counters[kCountBase + 8 * i] = 0;
     d.counters[kCountBase + 8 * i + 1] = 0;
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(256) void k_ingest(Dev d, const double *__restrict_
   if (i == 0) {
     *d.status = 0;
     for (int k = 0; k < 8; k++) d.counters[k] = 0;
-    for (int k = 0; k < Q_COUNT; k++) d.queue[k] = 0;
+
     if (d.skin > 0.f) {       // neighbour-list reuse: forced rebuilds
       const int calls = d.nbl_state[1] + 1;
       d.nbl_state[1] = calls;

@@ -375,7 +375,7 @@ __global__ __launch_bounds__(64) void k_tree(Dev d) {
   __shared__ TreeLds<CAP, PB> S;
   __shared__ int slot;
   const int count = (int)d.counters[6 + (PB ? 1 : 0)];
-  unsigned *head = &d.queue[PB ? Q_TREE_FB : Q_TREE_FA];
+  unsigned *head = queue_head(d.queue, PB ? Q_TREE_FB : Q_TREE_FA);
   unsigned long long tk = 0, te = 0;
   for (int k = first_item(); k < count; k = next_item(head, &slot))
     tree_dfs_root<CAP, PB>(d, S, d.tree_ovf[(size_t)(2 + (PB ? 1 : 0)) * d.nh + k], tk, te);
@@ -680,7 +680,7 @@ template <int CAP, bool PB>
 __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
   __shared__ BfsLds<CAP, PB, kLevelCap, 1> S;
   __shared__ int slot;
-  unsigned *head = &d.queue[PB ? Q_TREE_B : Q_TREE_A];
+  unsigned *head = queue_head(d.queue, PB ? Q_TREE_B : Q_TREE_A);
   // roots in runs of `run` per queue grab (one returning atomic on the queue
   // head per grab; at least two: the grabs serialise on that one address)
   const int run = max(2, min(8, d.nh / (int)(gridDim.x * 8)));
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(64 * kBigWaves) void k_tree_bfs_big(Dev d) {
   L &S = *reinterpret_cast<L *>(lds_raw);
   __shared__ int slot;
   const int count = (int)d.counters[4 + (PB ? 1 : 0)];
-  unsigned *head = &d.queue[PB ? Q_TREE_BB : Q_TREE_BA];
+  unsigned *head = queue_head(d.queue, PB ? Q_TREE_BB : Q_TREE_BA);
   unsigned long long tk = 0, te = 0;
   for (int k = first_item(); k < count; k = next_item(head, &slot))
     tree_bfs_root<CAP, PB, kLevelCapBig, kBigWaves>(d, S, d.tree_ovf[(size_t)(PB ? 1 : 0) * d.nh + k], 1, tk, te);
