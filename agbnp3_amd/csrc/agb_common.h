@@ -26,6 +26,8 @@ constexpr float kHbSwA = 0.65f, kHbSwB = 1.0f, kHbSwA0 = 0.6f;
 // overlap-tree kept / evaluated totals spread over kCountSlots 64-byte lines
 // (kCountBase + 8 s, + 1), so the blocks' adds at exit do not queue on one
 // address
+// atoms from which a call runs the water-site kernels on a second stream
+constexpr int kForkAtoms = 16384;
 constexpr int kCountSlots = 32, kCountBase = 8;
 constexpr int kCounters = kCountBase + 8 * kCountSlots;
 constexpr int kTblN = 512;              // spline nodes per table (y[511] = 0)

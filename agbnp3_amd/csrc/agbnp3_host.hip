@@ -680,9 +680,11 @@ int build_pack(Pack &P, const std::vector<const Model *> &ms, int options) {
   HIPCHK(hipMemset(d.nbl_state, 0, sizeof(int) * 4));
   HIPCHK(hipHostMalloc((void **)&P.hstage, (18 * nn + 7 * (size_t)P.nsys) * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
+  if (P.n >= kForkAtoms) {              // only packs that fork (streams share few hardware queues)
+    HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
+  }
   HIPCHK(prepare_pairs(P.ntypes));
   HIPCHK(prepare_tree());
   return build_items(P);
@@ -728,8 +730,9 @@ int run_pipeline(Pack &P, const double *x, const double *y, const double *z, dou
   mark("tree_volumes");
   launch_scale(d, s);
   mark("scale");
-  // with per-kernel timing every kernel stays on the one stream
-  const bool fork = !timed && P.side && d.nws > 0;
+  // with per-kernel timing every kernel stays on the one stream; below
+  // kForkAtoms atoms the fork and join cost more than the overlap saves
+  const bool fork = !timed && P.side && d.nws > 0 && d.n >= kForkAtoms;
   if (fork) {
     HIPCHK(hipEventRecord(P.ev_fork, s));
     HIPCHK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
