@@ -29,7 +29,10 @@ int agbnp3_set_options(int tag, int options);
    gradients [3*natoms], interleaved like float_i (*)[3]; energies[7] =
    egb, evdw, ecorr_vdw, ecav, ecorr_cav, ehb, mol_volume.  Enqueued on the
    instance stream and returns without waiting; call agbnp3_sync (or wait on
-   agbnp3_stream) before reading outputs.  Host-side errors (bad tag, HB
+   agbnp3_stream) before reading outputs.  (From 16384 atoms the water-site
+   kernels run on a second stream of the instance, joined into the instance
+   stream before the call's last kernels: waiting on the instance stream
+   covers them.)  Host-side errors (bad tag, HB
    topology) are reported immediately; device-side capacity errors by the
    next agbnp3_sync. */
 int agbnp3_ener_device(int tag, const double *x, const double *y, const double *z,
