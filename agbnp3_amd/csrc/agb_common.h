@@ -297,7 +297,7 @@ void launch_ws_place(const Dev &d, hipStream_t s);
 void launch_ws_free(const Dev &d, hipStream_t s);
 void launch_sg_front(const Dev &d, hipStream_t s);
 void launch_sg_grad(const Dev &d, hipStream_t s);
-void launch_sg_judge(const Dev &d, hipStream_t s);
+void launch_sg_audit(const Dev &d, hipStream_t s);
 hipError_t prepare_surrogate();
 void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t s);
 void launch_sg_infer(const SgModel &m, const float *X, int n, float *s_out, float *dsdF, int grid, hipStream_t s);

@@ -750,7 +750,7 @@ int run_pipeline(Pack &P, const double *x, const double *y, const double *z, dou
   mark("gbpair");
   launch_q2ab(d, s);
   mark("q2ab");
-  launch_sg_judge(d, s);
+  launch_sg_audit(d, s);
   launch_chain(d, t, s);
   mark("chain");
   launch_sg_grad(d, s);

@@ -784,12 +784,12 @@ __global__ __launch_bounds__(256) void k_sg_grad(Dev d) {
   }
 }
 
-// ------------------------------------------------------------- the judge
+// ------------------------------------------------------- the anchor audit
 // On anchor calls of a system that was anchored before: the first-order
 // change of Delta G_solv had the surrogate been used instead of the plain
 // kernel, sum_i (dE/ds_i) (s_i^surrogate - s_i^plain) with
 // dE/ds_i = -(332 df q2ab_i + abrw_i) / 4pi; then mark the system anchored.
-__global__ __launch_bounds__(256) void k_sg_judge(Dev d) {
+__global__ __launch_bounds__(256) void k_sg_audit(Dev d) {
   __shared__ double red[256];
   const int b = blockIdx.x;
   int *st = d.sg_state + 4 * b;
@@ -860,8 +860,8 @@ void launch_sg_pack(const float *p, unsigned short *w1, unsigned short *w2, unsi
   hipLaunchKernelGGL(k_sg_pack, dim3((n + 255) / 256), dim3(256), 0, s, p, w1, w2, w3, wt1, wt2, wt3);
 }
 
-void launch_sg_judge(const Dev &d, hipStream_t s) {
-  if (d.sg_on) hipLaunchKernelGGL(k_sg_judge, dim3(d.nsys), dim3(256), 0, s, d);
+void launch_sg_audit(const Dev &d, hipStream_t s) {
+  if (d.sg_on) hipLaunchKernelGGL(k_sg_audit, dim3(d.nsys), dim3(256), 0, s, d);
 }
 
 void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t s) {

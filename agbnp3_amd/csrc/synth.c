@@ -1,4 +1,4 @@
-/* Seeded synthetic globular solutes (config-0 recipe); see synth.h.
+/* Seeded synthetic globular solutes (the globular-solute recipe); see synth.h.
  *
  * Growth model
  *   backbone : self-avoiding chain, bond 1.53 A, bond angle U[104.5,114.5] deg,

@@ -5,7 +5,7 @@
  * rejection into a sphere at 0.1 atoms/A^3, with pendant heavy atoms
  * (carbonyl / hydroxyl O, sulfone O pairs) and hydrogens at 1.0 A that bring
  * the hydrogen fraction to ~50%.  Elements, radii, charges, non-polar and
- * hydrogen-bond parameters follow the config-0 recipe; the bond graph is
+ * hydrogen-bond parameters follow the globular-solute recipe; the bond graph is
  * returned as a symmetric connection table (NeighList layout: nne + flat
  * neighbour array).  The random stream is xoshiro256** seeded by SplitMix64,
  * with Box-Muller normals, so a seed gives the same solute on every host.

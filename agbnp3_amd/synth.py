@@ -1,4 +1,4 @@
-"""Seeded synthetic solutes (config-0 recipe) via libagbnp3_synth.so.
+"""Seeded synthetic solutes (the globular-solute recipe) via libagbnp3_synth.so.
 
 `generate(seed, natoms)` returns a `System`: the arguments of agbnp3_new()
 as numpy arrays (caller atom order) plus the connection table in NeighList

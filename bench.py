@@ -2,8 +2,8 @@
 
 Workload (one "step"): one agbnp3_ener evaluation -- neighbour-list rebuild,
 self volumes and surface areas, Born radii, GB energy, vdW, cavity and
-hydrogen-bond terms with all gradients -- of the 25,000-atom config-1
-synthetic solute (seed 20260025), inputs and outputs resident in HBM
+hydrogen-bond terms with all gradients -- of the 25,000-atom
+synthetic globular solute (seed 20260025), inputs and outputs resident in HBM
 (agbnp3_ener_device).  value = atoms x steps / wall seconds.
 
 Also reported: the roofline position of the dominant kernel (algorithmic
@@ -14,14 +14,15 @@ and ms_per_step_with_kernel_events gives that run's rate) and the reference CPU 
 (the reference library built from its own sources, OpenMP) timed on this
 host on a bounded sample of the same workload.
 
-Batched form (config 2), reported beside: 512 instances of log-uniform size
+Batched form (512-instance batch), reported beside: 512 instances of log-uniform size
 in [300, 6000] atoms (sizes seed 20262999, instance k seed 20262000 + k),
 evaluated as one batch replayed through a hipGraph after per-instance
 Gaussian coordinate perturbations (sigma 0.05 A, replay r seed 20263000 + r),
 against the same instances evaluated one agbnp3_ener_device call each.
 
-Learned surrogate (config 3), reported beside: a Born-radius MLP trained by
-the GPU kernels on ~2M atom rows of config-2-style conformations (sizes seed
+Learned surrogate (50 x 6000-atom batch), reported beside: a Born-radius MLP trained by
+the GPU kernels on ~2M atom rows of conformations drawn like the batched
+form's (sizes seed
 20264999, conformation m seed 20264000 + m, each as generated and displaced
 by sigma 0.05 A; radii from the plain kernels),
 then 50 held-out 6000-atom solutes (seeds 20265000 + k) evaluated as one
@@ -29,7 +30,7 @@ batch with the surrogate (plain anchor every 10th call) replayed through a
 hipGraph, against the plain per-instance calls on the same perturbed frames
 (sigma 0.05 A, frame r seed 20265100 + r); Delta G_solv error per solute.
 
-Trajectory (config 4), reported beside: the 150,000-atom solute (seed
+Trajectory (150,000-atom solute), reported beside: the 150,000-atom solute (seed
 20266150), frames of accumulated Gaussian displacements (sigma 0.03 A,
 frame f seed 20267000 + f), neighbour list kept with a 1 A skin and rebuilt
 every 10 frames, surrogate anchored every 10 frames, against the plain
@@ -133,7 +134,7 @@ class DeviceIO:
 
 
 def train_surrogate(args, api, hip, synth, np):
-    """Config 3 training set and training: plain radii of config-2-style conformations, GPU kernels."""
+    """Surrogate training set and training: plain radii of conformations drawn like the batched form's, GPU kernels."""
     t0 = time.perf_counter()
     model = api.Surrogate(seed=20264001)
     model.reserve(args.sg_train_atoms + 20000)
@@ -182,7 +183,7 @@ def dg_rel_err(e_a, e_b, np):
 
 
 def bench_surrogate(args, model, api, hip, synth, np):
-    """Config 3: held-out solutes, batched + surrogate (graph replays) vs plain per-instance calls."""
+    """50 held-out 6000-atom solutes, batched + surrogate (graph replays) vs plain per-instance calls."""
     ss = [synth.generate(20265000 + k, 6000) for k in range(args.c3_solutes)]
     insts = [api.Instance(g) for g in ss]
     b = api.Batch(insts)
@@ -280,7 +281,7 @@ def bench_surrogate(args, model, api, hip, synth, np):
     b.delete()
     for i in insts:
         i.delete()
-    return dict(workload=f"config 3: {len(ss)} held-out 6000-atom solutes (seeds 20265000+k) as one batch, "
+    return dict(workload=f"surrogate: {len(ss)} held-out 6000-atom solutes (seeds 20265000+k) as one batch, "
                          f"surrogate with a plain anchor every 10th call, neighbour lists kept (1 A skin), "
                          f"hipGraph replays of sigma 0.05 A frames",
                 natoms=int(n), calls=reps, surrogate_atom_evals_per_s=n / t_sur, surrogate_ms_per_call=1e3 * t_sur,
@@ -289,11 +290,11 @@ def bench_surrogate(args, model, api, hip, synth, np):
                 max_rel_dG_err=max(errs), rel_dG_err_per_call=errs,
                 systems_on_surrogate_last_call=int(np.sum(modes_last)), systems=int(b.nsys),
                 surrogate_kernel_ms=sur_kernels,
-                judge_abs_err_kcal=float(np.max(np.abs(err_est))))
+                audit_abs_err_kcal=float(np.max(np.abs(err_est))))
 
 
 def bench_trajectory(args, model, api, hip, synth, np):
-    """Config 4: 150k-atom trajectory, surrogate + neighbour-list reuse vs plain calls."""
+    """150k-atom trajectory, surrogate + neighbour-list reuse vs plain calls."""
     s = synth.generate(20266150, args.c4_natoms)
     n = s.natoms
     plain, sur = api.Instance(s), api.Instance(s)
@@ -334,7 +335,7 @@ def bench_trajectory(args, model, api, hip, synth, np):
             a.free()
     plain.delete()
     sur.delete()
-    return dict(workload=f"config 4: {n}-atom solute (seed 20266150), {len(frames)} frames of accumulated sigma "
+    return dict(workload=f"trajectory: {n}-atom solute (seed 20266150), {len(frames)} frames of accumulated sigma "
                          f"0.03 A displacements, neighbour list kept (1 A skin) and rebuilt every 10 frames, "
                          f"surrogate anchored every 10 frames",
                 natoms=n, frames=len(frames), surrogate_atom_evals_per_s=n / res["surrogate"],
@@ -344,7 +345,7 @@ def bench_trajectory(args, model, api, hip, synth, np):
 
 
 def bench_batched(args, api, hip, synth, np):
-    """Config 2: plain per-instance calls vs one batch replayed by hipGraph."""
+    """512-instance batch: plain per-instance calls vs one batch replayed by hipGraph."""
     sizes = synth.sizes(20262999, args.batch_instances, 300, 6000)
     ss = [synth.generate(20262000 + k, int(nk)) for k, nk in enumerate(sizes)]
     log(f"bench: batched workload, {len(ss)} instances, {int(sizes.sum())} atoms")
@@ -424,7 +425,7 @@ def bench_batched(args, api, hip, synth, np):
     b.delete()
     for inst in insts:
         inst.delete()
-    return dict(workload=f"config 2: {len(ss)} instances, sizes log-uniform in [300, 6000] (seed 20262999), "
+    return dict(workload=f"batched: {len(ss)} instances, sizes log-uniform in [300, 6000] (seed 20262999), "
                          f"{n} atoms per replay, hipGraph replay after sigma 0.05 A perturbations",
                 natoms=int(n), instances=len(ss), replays=args.batch_steps,
                 batched_atom_evals_per_s=n / t_batch, batched_ms_per_replay=1e3 * t_batch,
@@ -578,7 +579,7 @@ def main():
         metric=METRIC, value=value, unit="atom-evals/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f32 pairs, f64 reductions", data="synthetic",
-        config=dict(workload="config 1: plain agbnp3_ener E+grad with neighbour-list rebuild, one synthetic "
+        config=dict(workload="plain agbnp3_ener E+grad with neighbour-list rebuild, one synthetic "
                              f"globular solute of {n} atoms (seed {args.seed}), inputs resident in HBM",
                     natoms=n, nheavy=nh, instances=1),
         roofline=roofline, cpu_baseline=cpu,

@@ -152,7 +152,7 @@ int agbnp3_sg_grads(int model, int r0, int B, float *grad, float *loss);
 int agbnp3_sg_infer(int model, const float *F, int n, float *s, float *dsdF);
 int agbnp3_sg_attach(int tag, int model, int every);
 int agbnp3_batch_sg_attach(int batch, int model, int every);
-/* per system: error estimate of the last judged anchor, mode of the last call
+/* per system: error estimate of the last audited anchor, mode of the last call
    (1 = surrogate) */
 int agbnp3_sg_stats(int tag, double *err, int *mode);
 int agbnp3_batch_sg_stats(int batch, double *err, int *mode);

@@ -31,4 +31,4 @@ def load_golden(name):
     return s, ref
 
 
-GOLDEN_CASES = ("c0_n2000", "n600_s11", "n300_s7", "n400_corr")
+GOLDEN_CASES = ("globular_n2000", "n600_s11", "n300_s7", "n400_corr")

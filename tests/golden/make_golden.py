@@ -27,7 +27,7 @@ import oracle  # noqa: E402
 
 REF_DIR = os.environ.get("AGBNP3_REF_DIR", "/root/reference")
 CASES = [  # name, seed, natoms, correction-parameter variant
-    ("c0_n2000", 20260001, 2000, False),
+    ("globular_n2000", 20260001, 2000, False),
     ("n600_s11", 11, 600, False),
     ("n300_s7", 7, 300, False),
     ("n400_corr", 23, 400, True),

@@ -179,7 +179,7 @@ def test_batch_rejects_mixed_options():
         c.delete()
 
 
-def test_config2_shaped_batch():
+def test_512_instance_size_law_batch():
     """64 instances with log-uniform sizes in [300, 6000] (the batched
     workload's size law), each member bitwise equal to its own evaluation."""
     sizes = synth.sizes(20262999, 64, 300, 6000)

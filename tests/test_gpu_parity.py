@@ -71,7 +71,7 @@ def test_matches_reference_golden_vectors(golden_runs, case):
         assert rms(out[k] - ref[k]) <= 2e-3, (k, rms(out[k] - ref[k]))
 
 
-@pytest.mark.parametrize("case", ("c0_n2000", "n600_s11"))
+@pytest.mark.parametrize("case", ("globular_n2000", "n600_s11"))
 def test_neighbour_list_is_bit_exact(golden_runs, case):
     s, ref, inst, out, orc, oref = golden_runs[case]
     assert np.array_equal(inst.int2ext(), orc.int2ext())
@@ -93,7 +93,7 @@ def test_spline_tables_are_bit_exact(golden_runs, case):
 
 
 def test_internal_arrays_match_oracle(golden_runs):
-    s, ref, inst, out, orc, oref = golden_runs["c0_n2000"]
+    s, ref, inst, out, orc, oref = golden_runs["globular_n2000"]
     for name, tol in (("volumep_raw", 1e-5), ("sa_raw", 1e-4), ("sp", 1e-6), ("beta", 1e-6), ("A", 1e-5),
                       ("q2ab", 1e-4), ("deru", 1e-5), ("derv", 1e-7), ("derh", 1e-5)):
         g, o = inst.internal(name), orc.internal(name)
@@ -101,7 +101,7 @@ def test_internal_arrays_match_oracle(golden_runs):
 
 
 def test_overlap_tree_size_matches_oracle(golden_runs):
-    s, ref, inst, out, orc, oref = golden_runs["c0_n2000"]
+    s, ref, inst, out, orc, oref = golden_runs["globular_n2000"]
     cg, co = inst.counts(), orc.counts()
     assert cg["near_pairs"] == co["near_pairs"]
     assert cg["water_sites"] == co["water_sites"]
@@ -124,7 +124,7 @@ def test_corrected_modes_match_oracle(opts):
 
 
 def test_repeated_calls_are_bitwise_identical(golden_runs):
-    s, ref, inst, out, orc, oref = golden_runs["c0_n2000"]
+    s, ref, inst, out, orc, oref = golden_runs["globular_n2000"]
     again = inst.ener()
     for k in TERMS:
         assert again[k] == out[k], k
@@ -189,7 +189,7 @@ def test_moderate_system_matches_oracle():
     check_against_oracle(out, oracle.Oracle(s).ener(), label="n6000")
 
 
-def test_config1_size_invariants():
+def test_25k_atom_size_invariants():
     # 25,000 atoms: size-independent properties (the FP64 oracle is too slow here)
     s = synth.generate(20260025, 25000)
     inst = api.Instance(s)

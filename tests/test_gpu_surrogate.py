@@ -264,7 +264,7 @@ def test_surrogate_gradients_are_derivatives_of_its_energy(small_set):
 
 
 def test_training_and_the_gain_over_frozen_radii():
-    """Train on config-2-style conformations (seeds 20264000 + m) with the
+    """Train on conformations of the batch size law (300 to 6000 atoms) (seeds 20264000 + m) with the
     GPU kernels; on a held-out solute, between anchors the network tracks
     the plain Delta G_solv better than keeping the anchor's radii."""
     sizes = synth.sizes(20264999, 24, 300, 3000)

@@ -59,7 +59,7 @@ def numpy_near_list(s):
     return order, rows
 
 
-@pytest.mark.parametrize("case", ("n600_s11", "c0_n2000"))
+@pytest.mark.parametrize("case", ("n600_s11", "globular_n2000"))
 def test_oracle_near_list_exact(case):
     s, _ = load_golden(case)
     o = oracle.Oracle(s)

@@ -1,4 +1,4 @@
-"""The config-0 generator: determinism, composition, packing, HB topology."""
+"""The globular-solute generator: determinism, composition, packing, HB topology."""
 import numpy as np
 
 from agbnp3_amd import synth
