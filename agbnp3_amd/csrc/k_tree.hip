@@ -696,11 +696,13 @@ __global__ __launch_bounds__(64) void k_tree_bfs(Dev d) {
 }
 
 // second tier: the roots the first tier queued, with 1536-node levels in
-// dynamic LDS (one block per CU) shared by kBigWaves waves; what still
+// dynamic LDS (one block per CU) shared by big_waves(pass) waves (pass B's
+// registers allow 8 per block, pass A's 16); what still
 // overflows goes to the depth-first kernel
-constexpr int kBigWaves = 8;
+constexpr int big_waves(bool pb) { return pb ? 8 : 16; }
 template <int CAP, bool PB>
-__global__ __launch_bounds__(64 * kBigWaves) void k_tree_bfs_big(Dev d) {
+__global__ __launch_bounds__(64 * big_waves(PB)) void k_tree_bfs_big(Dev d) {
+  constexpr int kBigWaves = big_waves(PB);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   using L = BfsLds<CAP, PB, kLevelCapBig, kBigWaves>;
   L &S = *reinterpret_cast<L *>(lds_raw);
@@ -714,7 +716,7 @@ __global__ __launch_bounds__(64 * kBigWaves) void k_tree_bfs_big(Dev d) {
 }
 
 template <int CAP, bool PB>
-static size_t big_lds() { return sizeof(BfsLds<CAP, PB, kLevelCapBig, kBigWaves>); }
+static size_t big_lds() { return sizeof(BfsLds<CAP, PB, kLevelCapBig, big_waves(PB)>); }
 
 hipError_t prepare_tree() {
   hipError_t e = hipSuccess;
@@ -738,11 +740,11 @@ static void launch_tree_cap(const Dev &d, int pass, hipStream_t s) {
   const size_t la = big_lds<CAP, false>(), lb = big_lds<CAP, true>();
   if (pass == 0) {
     hipLaunchKernelGGL((k_tree_bfs<CAP, false>), dim3(g), dim3(64), 0, s, d);
-    hipLaunchKernelGGL((k_tree_bfs_big<CAP, false>), dim3(gf), dim3(64 * kBigWaves), la, s, d);
+    hipLaunchKernelGGL((k_tree_bfs_big<CAP, false>), dim3(gf), dim3(64 * big_waves(false)), la, s, d);
     hipLaunchKernelGGL((k_tree<CAP, false>), dim3(gf), dim3(64), 0, s, d);
   } else {
     hipLaunchKernelGGL((k_tree_bfs<CAP, true>), dim3(g), dim3(64), 0, s, d);
-    hipLaunchKernelGGL((k_tree_bfs_big<CAP, true>), dim3(gf), dim3(64 * kBigWaves), lb, s, d);
+    hipLaunchKernelGGL((k_tree_bfs_big<CAP, true>), dim3(gf), dim3(64 * big_waves(true)), lb, s, d);
     hipLaunchKernelGGL((k_tree<CAP, true>), dim3(gf), dim3(64), 0, s, d);
   }
 }
