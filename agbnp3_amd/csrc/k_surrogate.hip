@@ -383,12 +383,13 @@ __device__ __forceinline__ void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
   lo = (__bf16)(v - (float)hi);
 }
 
-__device__ void stage_weights(const SgModel &m, MlpLds &L) {
-  for (int k = threadIdx.x; k < kSgHid * kSgKPad; k += 256) {
+template <typename Lds, int NT = 256>
+__device__ void stage_weights(const SgModel &m, Lds &L) {
+  for (int k = threadIdx.x; k < kSgHid * kSgKPad; k += NT) {
     const int o = k / kSgKPad, c = k - o * kSgKPad;
     L.wt1[o * kLd1 + c] = __builtin_bit_cast(__bf16, m.wt1[k]);
   }
-  for (int k = threadIdx.x; k < kSgHid * kSgHid; k += 256) {
+  for (int k = threadIdx.x; k < kSgHid * kSgHid; k += NT) {
     const int o = k / kSgHid, c = k - o * kSgHid;
     L.wt2[o * kLdH + c] = __builtin_bit_cast(__bf16, m.wt2[k]);
     L.wt3[o * kLdH + c] = __builtin_bit_cast(__bf16, m.wt3[k]);
@@ -470,32 +471,194 @@ __device__ void mlp_backward_tile(const BackW &B, MlpLds &L, bool want_dx, Keep 
   }
 }
 
-// inference over the atoms of systems with a model: s_i and ds_i/dF_i
-__global__ __launch_bounds__(256) void k_sg_mlp(Dev d) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
-  const SgModel &m = d.sg;
-  stage_weights(m, L);
-  BackW BW;
-  load_backw(m, BW);
-  const int ntiles = (d.n + kSgTile - 1) / kSgTile;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int r0 = tile * kSgTile, nrows = min(kSgTile, d.n - r0);
-    __syncthreads();
-    const float y = mlp_forward_tile(m, L, d.sg_X, r0, nrows);
-    if (threadIdx.x < kSgTile) {
-      if (threadIdx.x < nrows) d.sg_s[r0 + threadIdx.x] = y * m.ysd + m.ymu;
-      L.seed()[threadIdx.x] = 1.0f;
-    }
-    __syncthreads();
-    mlp_seed_delta3(m, L);
-    mlp_backward_tile(BW, L, true, [](int) {});
-    for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
-      const int r = k >> 6, c = k & 63;
-      if (r < nrows)                  // channel-major: a row's neighbours read adjacent 64-byte slices
-        d.sg_dsdF[((size_t)(c >> 4) * d.n + r0 + r) * kSgRow + kSgPad + (c & 15)] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
+// ------------------------------------------------ inference, 64-atom tiles
+// Inference needs no stored activations, so each wave keeps its own output
+// tiles in registers through the whole tile: with 8 waves, wave w owns
+// column tile w of every 128-wide layer for all 64 rows (4 row tiles), so
+// z1..z3, the SiLU derivative factors and the backward products line up lane
+// by lane and only the bf16 A operands of the next product pass through LDS.
+// The next tile's inputs are loaded while the current one runs.
+constexpr int kSgInfTile = 64, kSgInfWaves = 8, kSgInfThreads = 64 * kSgInfWaves;
+constexpr int kSgInfLoads = kSgInfTile * kSgKPad / kSgInfThreads;   // input floats per thread and tile
+struct InferLds {
+  __bf16 wt1[kSgHid * kLd1], wt2[kSgHid * kLdH], wt3[kSgHid * kLdH];
+  __bf16 act[kSgInfTile * kLdH], alo[kSgInfTile * kLdH];
+  float part[kSgInfTile][kSgInfWaves];
+};
+static_assert(sizeof(InferLds) <= 160 * 1024, "MLP inference tile state exceeds the LDS of one workgroup");
+static_assert(kSgInfTile * kSgKPad % kSgInfThreads == 0, "input tile is not a whole number of loads per thread");
+static_assert(kSgInfWaves * 16 == kSgHid, "one 16-column tile per wave");
+
+typedef f32x4 InfAcc[4];              // [row tile] of this wave's column tile
+
+// z = A W + bias over this wave's tiles; A = Ahi + Alo (bf16 pair), W^T in LDS
+template <int K, int LD>
+__device__ __forceinline__ void infer_layer(const __bf16 *Ahi, const __bf16 *Alo, const __bf16 *Wt, float bias, InfAcc &z) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) z[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bf16x8 b = *(const bf16x8 *)(Wt + (w * 16 + (lane & 15)) * LD + k0 + 8 * (lane >> 4));
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const int ao = (mt * 16 + (lane & 15)) * LD + k0 + 8 * (lane >> 4);
+      z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8 *)(Ahi + ao), b, z[mt], 0, 0, 0);
+      z[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8 *)(Alo + ao), b, z[mt], 0, 0, 0);
     }
   }
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) z[mt][r] += bias;
+}
+
+// element (row tile mt, row r of the lane) of this wave's column tile in an LDS tile
+__device__ __forceinline__ int infer_at(int mt, int r) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  return (mt * 16 + 4 * (lane >> 4) + r) * kLdH + w * 16 + (lane & 15);
+}
+
+// silu(z) as the bf16 pair of the next layer's A operand
+__device__ __forceinline__ void infer_put_act(InferLds &L, const InfAcc &z) {
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) split_bf16(silu(z[mt][r]), L.act[infer_at(mt, r)], L.alo[infer_at(mt, r)]);
+}
+
+// delta (bf16, into act) = g * silu'(z) lane by lane
+__device__ __forceinline__ void infer_put_delta(InferLds &L, const InfAcc &g, const InfAcc &z) {
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) L.act[infer_at(mt, r)] = (__bf16)(g[mt][r] * silup(z[mt][r]));
+}
+
+// g = delta W^T over this wave's column tile of a 128-wide layer; the B
+// operands are rows of the row-major W (read through the caches)
+__device__ __forceinline__ void infer_back128(const __bf16 *D, const unsigned short *W, InfAcc &g) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  bf16x8 b[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) b[kk] = back_operand(W, w, kk);
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8 *)(D + (mt * 16 + (lane & 15)) * kLdH + kk * 32 + 8 * (lane >> 4)),
+                                                    b[kk], acc, 0, 0, 0);
+    g[mt] = acc;
+  }
+}
+
+// Forward and input gradient of every 64-row tile of X (rows [0, n)), by a
+// block of kSgInfThreads: put_s(row, s) gets the denormalised output,
+// put_dx(row, c, v) the derivative ds/dF_c of the 64 shell features (rows
+// >= n reach neither).
+template <typename PutS, typename PutDx>
+__device__ void mlp_infer_rows(const SgModel &m, InferLds &L, const float *X, int n, PutS put_s, PutDx put_dx) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  stage_weights<InferLds, kSgInfThreads>(m, L);
+  const int c = w * 16 + (lane & 15);            // this lane's column of the 128-wide layers
+  const float bias1 = m.b1[c], bias2 = m.b2[c], bias3 = m.b3[c], w4c = m.w4[c], b4 = *m.b4;
+  // the input gradient's 64 columns: waves w and w + 4 share column tile w % 4, row tiles 2(w / 4) + {0, 1}
+  const int cx = (w & 3) * 16 + (lane & 15), mx = 2 * (w >> 2);
+  const float dxs = m.ysd * m.xsd_inv[cx];
+  bf16x8 bw1[4];
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) bw1[kk] = back_operand(m.w1, w & 3, kk);
+  const int ntiles = (n + kSgInfTile - 1) / kSgInfTile;
+  float xv[kSgInfLoads];
+  auto load_x = [&](int tile) {
+#pragma unroll
+    for (int j = 0; j < kSgInfLoads; j++) {
+      const int k = threadIdx.x + kSgInfThreads * j, r = tile * kSgInfTile + k / kSgKPad;
+      xv[j] = r < n ? X[(size_t)r * kSgKPad + k % kSgKPad] : 0.f;
+    }
+  };
+  if ((int)blockIdx.x < ntiles) load_x(blockIdx.x);
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int r0 = tile * kSgInfTile;
+    __syncthreads();                             // the previous tile's last products have read act
+#pragma unroll
+    for (int j = 0; j < kSgInfLoads; j++) {
+      const int k = threadIdx.x + kSgInfThreads * j, o = (k / kSgKPad) * kLd1 + k % kSgKPad;
+      split_bf16(xv[j], L.act[o], L.alo[o]);
+    }
+    if (tile + (int)gridDim.x < ntiles) load_x(tile + gridDim.x);
+    __syncthreads();
+    InfAcc z1, z2, z3, g;
+    infer_layer<kSgKPad, kLd1>(L.act, L.alo, L.wt1, bias1, z1);
+    __syncthreads();
+    infer_put_act(L, z1);
+    __syncthreads();
+    infer_layer<kSgHid, kLdH>(L.act, L.alo, L.wt2, bias2, z2);
+    __syncthreads();
+    infer_put_act(L, z2);
+    __syncthreads();
+    infer_layer<kSgHid, kLdH>(L.act, L.alo, L.wt3, bias3, z3);
+    __syncthreads();
+    // output partials over this wave's 16 columns (fixed-order butterfly in
+    // each 16-lane row group), and delta3 = w4 silu'(z3) for the backward
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        float v = silu(z3[mt][r]) * w4c;
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if ((lane & 15) == 0) L.part[mt * 16 + 4 * (lane >> 4) + r][w] = v;
+        g[mt][r] = w4c;
+      }
+    infer_put_delta(L, g, z3);
+    __syncthreads();
+    if (threadIdx.x < kSgInfTile && r0 + (int)threadIdx.x < n) {
+      const float *p = L.part[threadIdx.x];
+      float y = b4;
+#pragma unroll
+      for (int k = 0; k < kSgInfWaves; k++) y += p[k];
+      put_s(r0 + threadIdx.x, y * m.ysd + m.ymu);
+    }
+    infer_back128(L.act, m.w3, g);
+    __syncthreads();
+    infer_put_delta(L, g, z2);
+    __syncthreads();
+    infer_back128(L.act, m.w2, g);
+    __syncthreads();
+    infer_put_delta(L, g, z1);
+    __syncthreads();
+    // ds/dF of the 64 shell features
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int mt = mx + q;
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            *(const bf16x8 *)(L.act + (mt * 16 + (lane & 15)) * kLdH + kk * 32 + 8 * (lane >> 4)), bw1[kk], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = r0 + mt * 16 + 4 * (lane >> 4) + r;
+        if (row < n) put_dx(row, cx, acc[r] * dxs);
+      }
+    }
+  }
+}
+
+// inference over the atoms of systems with a model: s_i and ds_i/dF_i
+__global__ __launch_bounds__(kSgInfThreads) void k_sg_mlp(Dev d) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  InferLds &L = *reinterpret_cast<InferLds *>(lds_raw);
+  float *s = d.sg_s, *dsdF = d.sg_dsdF;
+  const size_t n = d.n;
+  mlp_infer_rows(
+      d.sg, L, d.sg_X, d.n, [=](int r, float v) { s[r] = v; },
+      // channel-major: a row's neighbours read adjacent 64-byte slices
+      [=](int r, int c, float v) { dsdF[((size_t)(c >> 4) * n + r) * kSgRow + kSgPad + (c & 15)] = v; });
 }
 
 // ---------------------------------------------------------------- training
@@ -825,7 +988,7 @@ __global__ __launch_bounds__(256) void k_sg_append(Dev d, float *F, float *targe
 }
 
 // ----------------------------------------------------------- launchers
-size_t sg_mlp_lds() { return sizeof(MlpLds); }
+size_t sg_mlp_lds() { return std::max(sizeof(MlpLds), sizeof(InferLds)); }
 
 
 void launch_sg_front(const Dev &d, hipStream_t s) {
@@ -836,8 +999,8 @@ void launch_sg_front(const Dev &d, hipStream_t s) {
   hipLaunchKernelGGL(k_sg_shadow, dim3((d.n + 255) / 256), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_features, dim3(d.nsg_rows), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_mode, dim3((d.nsys + 63) / 64), dim3(64), 0, s, d);
-  const int ntiles = (d.n + kSgTile - 1) / kSgTile;
-  hipLaunchKernelGGL(k_sg_mlp, dim3(std::min(ntiles, d.sg_grid)), dim3(256), sizeof(MlpLds), s, d);
+  const int ntiles = (d.n + kSgInfTile - 1) / kSgInfTile;
+  hipLaunchKernelGGL(k_sg_mlp, dim3(std::min(ntiles, d.sg_grid)), dim3(kSgInfThreads), sizeof(InferLds), s, d);
 }
 
 void launch_sg_train_step(const SgModel &m, const SgTrain &t, int r0, int B, float *grad, float *loss,
@@ -869,34 +1032,17 @@ void launch_sg_append(const Dev &d, float *F, float *target, int at, hipStream_t
 }
 
 // inference on given inputs (tests): X [n][kSgKPad] normalised FP32
-__global__ __launch_bounds__(256) void k_sg_infer(SgModel m, const float *X, int n, float *s_out, float *dsdF) {
+__global__ __launch_bounds__(kSgInfThreads) void k_sg_infer(SgModel m, const float *X, int n, float *s_out, float *dsdF) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  MlpLds &L = *reinterpret_cast<MlpLds *>(lds_raw);
-  stage_weights(m, L);
-  BackW BW;
-  load_backw(m, BW);
-  const int ntiles = (n + kSgTile - 1) / kSgTile;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int r0 = tile * kSgTile, nrows = min(kSgTile, n - r0);
-    __syncthreads();
-    const float y = mlp_forward_tile(m, L, X, r0, nrows);
-    if (threadIdx.x < kSgTile) {
-      if (threadIdx.x < nrows) s_out[r0 + threadIdx.x] = y * m.ysd + m.ymu;
-      L.seed()[threadIdx.x] = 1.0f;
-    }
-    __syncthreads();
-    mlp_seed_delta3(m, L);
-    mlp_backward_tile(BW, L, true, [](int) {});
-    for (int k = threadIdx.x; k < kSgTile * 64; k += 256) {
-      const int r = k >> 6, c = k & 63;
-      if (r < nrows) dsdF[(size_t)(r0 + r) * 64 + c] = L.z3[r * kSgHid + c] * m.ysd * m.xsd_inv[c];
-    }
-  }
+  InferLds &L = *reinterpret_cast<InferLds *>(lds_raw);
+  mlp_infer_rows(
+      m, L, X, n, [=](int r, float v) { s_out[r] = v; },
+      [=](int r, int c, float v) { dsdF[(size_t)r * 64 + c] = v; });
 }
 
 void launch_sg_infer(const SgModel &m, const float *X, int n, float *s_out, float *dsdF, int grid, hipStream_t s) {
-  const int ntiles = (n + kSgTile - 1) / kSgTile;
-  hipLaunchKernelGGL(k_sg_infer, dim3(std::min(ntiles, grid)), dim3(256), sizeof(MlpLds), s, m, X, n, s_out, dsdF);
+  const int ntiles = (n + kSgInfTile - 1) / kSgInfTile;
+  hipLaunchKernelGGL(k_sg_infer, dim3(std::min(ntiles, grid)), dim3(kSgInfThreads), sizeof(InferLds), s, m, X, n, s_out, dsdF);
 }
 
 void launch_sg_grad(const Dev &d, hipStream_t s) {
@@ -907,13 +1053,13 @@ void launch_sg_grad(const Dev &d, hipStream_t s) {
 
 hipError_t prepare_surrogate() {
   hipError_t e = hipFuncSetAttribute((const void *)k_sg_mlp, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)sizeof(MlpLds));
+                                     (int)sizeof(InferLds));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void *)k_sg_train_fb, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sizeof(MlpLds));
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void *)k_sg_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)sizeof(MlpLds));
+                            (int)sizeof(InferLds));
   return e;
 }
 
