@@ -215,9 +215,12 @@ __global__ __launch_bounds__(256) void k_sg_shadow_w(Dev d) {
 // the 4 shells its B-spline covers.  The bins hold fixed point (2^-24, sums
 // below 128) and take integer atomics: LDS float atomics run far slower on
 // this GPU, and integer sums do not depend on the order of the adds.
+// Each channel's 16 bins sit between kSgPad pad bins on either side (rows of
+// kSgRow), so the 4 shells of every pair take their adds without a range
+// test; the pads are never read.
 constexpr float kSgBinScale = 16777216.0f;      // 2^24
 __global__ __launch_bounds__(256) void k_sg_features(Dev d) {
-  __shared__ unsigned bins[4][64];
+  __shared__ unsigned bins[4][4 * kSgRow];
   const int rb = xcd_block(blockIdx.x, gridDim.x);
   const int2 blk = d.sg_rows[rb];
   const int sys = d.asys[blk.x];
@@ -232,15 +235,20 @@ __global__ __launch_bounds__(256) void k_sg_features(Dev d) {
     const int nh = over ? 0 : d.sg_nn[i].x;
     const int *nb = d.sg_nbr + ((size_t)rb * kSgBlk + r) * kSgNbrCap;
     bin[lane] = 0u;
-    // the next step's neighbour position and weight are loaded one step ahead
+    if (lane < 4 * kSgRow - 64) bin[64 + lane] = 0u;
+    // the next step's neighbour position and weight are loaded one step
+    // ahead, its list entry two steps ahead
     int un = lane < nh ? nb[lane] : t;
+    int unn = lane + 64 < nh ? nb[lane + 64] : t;
     float4 pn = d.sg_spos[un];
     float gn = d.sg_sgam[un];
     for (int k0 = 0; k0 < nh; k0 += 64) {
       const int k = k0 + lane;
       const float4 pj = pn;
       const float gj = gn;
-      un = k + 64 < nh ? nb[k + 64] : t;
+      un = unn;
+      const int e2 = nb[min(k + 128, nh - 1)];           // always a valid entry: no branch around the load
+      unn = k + 128 < nh ? e2 : t;
       pn = d.sg_spos[un];
       gn = d.sg_sgam[un];
       if (k >= nh) continue;
@@ -252,14 +260,12 @@ __global__ __launch_bounds__(256) void k_sg_features(Dev d) {
       sg_taper(dist, T, Tp);
       T *= gj * kSgBinScale;
       const int k0s = sg_basis(dist, b, db);
-      unsigned *f = bin + sg_channel(pj.w) * 16;
+      // shells k0s .. k0s + 3 with -kSgPad < k0s and k0s + 3 < kSgNShell + kSgPad
+      unsigned *f = bin + sg_channel(pj.w) * kSgRow + kSgPad + k0s;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int kk = k0s + q;
-        if (kk >= 0 && kk < kSgNShell) atomicAdd(&f[kk], (unsigned)__float2uint_rn(b[q] * T));
-      }
+      for (int q = 0; q < 4; q++) atomicAdd(&f[q], (unsigned)__float2uint_rn(b[q] * T));
     }
-    const float raw = (float)bin[lane] * (1.0f / kSgBinScale);
+    const float raw = (float)bin[(lane >> 4) * kSgRow + kSgPad + (lane & 15)] * (1.0f / kSgBinScale);
     // raw shell sums, normalised input row, in-range test (slot order)
     d.sg_F[(size_t)t * 64 + lane] = raw;
     float v = (raw - d.sg.xmu[lane]) * d.sg.xsd_inv[lane];
