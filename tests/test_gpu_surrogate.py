@@ -153,6 +153,20 @@ def test_mlp_inference_matches_cpu_reference(small_set):
     assert np.sqrt(np.mean((d_gpu - d_ref) ** 2)) <= 1e-2 * np.sqrt(np.mean(d_ref ** 2))
 
 
+def test_mlp_inference_rows_do_not_depend_on_their_tile(small_set):
+    """Each row's output and input gradient come from that row alone, so any
+    prefix of the rows (whole tiles, a partial last tile, a single row, a
+    row shifted to another position in its tile) reproduces the full call's rows bitwise."""
+    ss, m = small_set
+    F, _ = m.dataset()
+    s_all, d_all = m.infer(F)
+    for n in (1, 63, 64, 65, 130):
+        s, d = m.infer(F[:n])
+        assert np.array_equal(s, s_all[:n]) and np.array_equal(d, d_all[:n]), n
+    s, d = m.infer(F[37:37 + 70])
+    assert np.array_equal(s, s_all[37:107]) and np.array_equal(d, d_all[37:107])
+
+
 def compare_training_gradients(m, w_abs, w_diff):
     F, t = m.dataset()
     rows, c = m.train_rows()
