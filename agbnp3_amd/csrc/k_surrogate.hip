@@ -86,26 +86,24 @@ __device__ __forceinline__ int sg_channel(float R) {
 // order, so their order (and every FP32 sum over them) is reproducible.
 constexpr float kSgCellEdge = 4.5f;
 
-// per system: rebuild on the first call and after a move beyond kSgSkin / 2
+// Per system: rebuild on the first call and after a move beyond kSgSkin / 2.
+// The flag (sg_nbl[4b + 1]) starts as "never built" per system, then one
+// thread per atom of a built system raises it on such a move (every writer
+// stores the same 1), so a 150,000-atom system is checked by the whole GPU.
 __global__ __launch_bounds__(256) void k_sg_guard(Dev d) {
-  const int b = blockIdx.x;
-  if (!d.sys_sg[b]) return;                         // block-uniform
-  int *st = d.sg_nbl + 4 * b;
-  const SysInfo S = d.sys[b];
-  const bool built = st[0] != 0;
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= d.nsys || !d.sys_sg[b]) return;
+  d.sg_nbl[4 * b + 1] = d.sg_nbl[4 * b] == 0;
+}
+__global__ __launch_bounds__(256) void k_sg_moved(Dev d) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= d.n) return;
+  const int b = d.asys[i];
+  if (!d.sys_sg[b] || d.sg_nbl[4 * b] == 0) return;
   const float lim2 = 0.25f * kSgSkin * kSgSkin;
-  int moved = !built;
-  if (built)
-    for (int i = S.a0 + threadIdx.x; i < S.a0 + S.n; i += 256) {
-      const float4 p = d.pos[i], r = d.sg_ref[i];
-      const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
-      moved |= !(ex * ex + ey * ey + ez * ez <= lim2);   // non-finite counts as moved
-    }
-  moved = __syncthreads_or(moved);
-  if (threadIdx.x == 0) {
-    st[1] = moved;
-    if (moved) st[2] += 1;
-  }
+  const float4 p = d.pos[i], r = d.sg_ref[i];
+  const float ex = p.x - r.x, ey = p.y - r.y, ez = p.z - r.z;
+  if (!(ex * ex + ey * ey + ez * ez <= lim2)) d.sg_nbl[4 * b + 1] = 1;   // non-finite counts as moved
 }
 
 // all atoms of a rebuilt system into cells of edge >= kSgCellEdge
@@ -117,6 +115,7 @@ __global__ __launch_bounds__(1024) void k_sg_cells(Dev d) {
   const int b = blockIdx.x;
   if (!d.sys_sg[b] || !d.sg_nbl[4 * b + 1]) return;   // block-uniform
   const SysInfo S = d.sys[b];
+  if (threadIdx.x == 0) d.sg_nbl[4 * b + 2] += 1;     // builds so far
   float lo[3], hi[3];
   block_bbox(d.pos, S.a0, S.n, box, lo, hi);
   if (threadIdx.x == 0) {
@@ -999,7 +998,8 @@ size_t sg_mlp_lds() { return std::max(sizeof(MlpLds), sizeof(InferLds)); }
 
 void launch_sg_front(const Dev &d, hipStream_t s) {
   if (!d.sg_on) return;
-  hipLaunchKernelGGL(k_sg_guard, dim3(d.nsys), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_sg_guard, dim3((d.nsys + 255) / 256), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_sg_moved, dim3((d.n + 255) / 256), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_cells, dim3(d.nsys), dim3(1024), 0, s, d);
   hipLaunchKernelGGL(k_sg_nbl, dim3(d.nsg_rows), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_sg_shadow, dim3((d.n + 255) / 256), dim3(256), 0, s, d);
