@@ -234,25 +234,17 @@ __global__ __launch_bounds__(256) void k_energy(Dev d, double *energies) {
   const SysInfo S = d.sys[blockIdx.x];
   double acc[EP_COUNT];
   for (int k = 0; k < EP_COUNT; k++) acc[k] = 0.0;
-  // contiguous ranges per thread keep the order fixed
-  auto range = [&](int n, int &b0, int &b1) {
-    const int per = (n + 255) / 256;
-    b0 = min(n, t * per);
-    b1 = min(n, b0 + per);
-  };
-  int b0, b1;
-  range(S.nblk, b0, b1);
-  // (the loops unrolled so that several loads are in flight; each sum keeps its order)
+  // thread t sums partials t, t + 256, t + 512, ... in that order: a fixed
+  // order that depends only on the system's own counts, with coalesced loads
+  // (the loops unrolled so that several loads are in flight)
   for (int k = 0; k < EP_PAIR; k++)
 #pragma unroll 8
-    for (int b = b0; b < b1; b++) acc[k] += d.epart[(size_t)k * d.nablk + S.blk0 + b];
-  range(S.ntile, b0, b1);
+    for (int b = t; b < S.nblk; b += 256) acc[k] += d.epart[(size_t)k * d.nablk + S.blk0 + b];
   // tile partials hold each unordered pair once; E_pair = 2 df sum_{i<j} q_i q_j / f
 #pragma unroll 8
-  for (int b = b0; b < b1; b++) acc[EP_PAIR] += 2.0 * S.df64 * d.gb_epart[S.tile0 + b];
-  range(S.nws, b0, b1);
+  for (int b = t; b < S.ntile; b += 256) acc[EP_PAIR] += 2.0 * S.df64 * d.gb_epart[S.tile0 + b];
 #pragma unroll 8
-  for (int b = b0; b < b1; b++) acc[EP_HB] += d.ws_part[S.ws0 + b];
+  for (int b = t; b < S.nws; b += 256) acc[EP_HB] += d.ws_part[S.ws0 + b];
   for (int k = 0; k < EP_COUNT; k++) {
     const double s = wave_sum(acc[k]);
     if (lane == 0) red[k][w] = s;
