@@ -20,10 +20,12 @@
 // bf16; inputs and hidden activations are carried as a bf16 pair (hi + lo,
 // about 16 significant bits, two products per layer), so the model is a
 // smooth function of the coordinates rather than one that steps with bf16
-// rounding.  One block (4 waves) holds the three weight matrices in LDS and
-// runs 32-atom tiles: forward, then the input gradient ds/dF back through
-// the layers with bf16 deltas (the transposed products read the row-major
-// weights from L2).
+// rounding.  A block holds the three weight matrices in LDS and runs atom
+// tiles: forward, then the input gradient ds/dF back through the layers with
+// bf16 deltas (the transposed products read the row-major weights through
+// the caches).  Inference runs 64-atom tiles on 8 waves that keep their
+// output tiles in registers; training runs 32-atom tiles on 4 waves through
+// LDS, since it stores every activation and delta.
 //
 // MFMA 16x16x32 lane layout: lane l holds A[l%16][8(l/16)+r],
 // B[8(l/16)+r][l%16] (r = 0..7) and D[4(l/16)+r][l%16] (r = 0..3).
